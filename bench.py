@@ -1,0 +1,145 @@
+"""Throughput of the Monte Carlo fault-tree engine on one MI355X.
+
+Workload: plain Monte Carlo on the large synthetic layered tree (25,000 basic
+events with p log-uniform in [1e-6, 1e-2]; 40,000 gates in 30 layers, fan-in
+2-8, 48 % AND / 47 % OR / 5 % ATLEAST with n up to 16, 10 % negated inputs),
+every gate tallied, 2^18 uint64 words = 2^24 trials per pass. One step is one
+pass (Philox sampling of every basic event, every gate layer, tallies);
+2^32 trials are 256 steps.
+
+Prints one JSON line: gate-evals/s (gates x trials / s), the HBM roofline of
+the dominant kernel measured with HIP events on the engine's stream, and the
+CPU oracle timed on the host on a bounded sample of the same tree (with a
+bit-exact tally comparison against the GPU on that sample).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+HBM_PEAK_GBS = 8000.0
+
+
+def algorithmic_bytes(pd, n_bp: int) -> dict:
+    """Bytes the materialised dataflow moves per pass: each node word written
+    once, each gate input word read once, each tallied word read once."""
+    n_be, n_g, f = pd.n_vars, pd.n_gates, pd.n_args
+    return {"sampler": 8 * n_be * n_bp,
+            "gates": 8 * (n_g + f) * n_bp,
+            "pass": 8 * (n_be + n_g + f + n_g) * n_bp}
+
+
+def cpu_baseline(pd, target_s: float, max_words: int):
+    from oracle import binding as orc
+    tally = pd.gate_index.astype(np.int32)
+    t = time.perf_counter()
+    orc.run(pd, 256, tally=tally)
+    per_word = (time.perf_counter() - t) / 256
+    words = 256
+    while words * 2 <= max_words and per_word * words * 2 <= target_s:
+        words *= 2
+    t = time.perf_counter()
+    ones, _ = orc.run(pd, words, tally=tally)
+    dt = time.perf_counter() - t
+    return words, ones, dt, orc.max_threads()
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=32)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--words", type=int, default=1 << 18, help="uint64 words per pass")
+    ap.add_argument("--tree", choices=["large", "small"], default="large")
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU: the engine's sharding over trials is a "
+              "design note, not a launch path; run with --gpus 1", file=sys.stderr)
+        return 2
+
+    import mcscram_amd as mc
+    pd = mc.large_tree() if args.tree == "large" else mc.small_tree(1)
+    tally = [int(g) for g in pd.gate_index]
+    n_bp = args.words
+    eng = mc.LayerManager(pd, n_bp, tally=tally)
+    info = eng.info
+    if args.warmup:
+        eng.time_passes(args.warmup)
+    eng.wait()
+    t0 = time.perf_counter()
+    dev = eng.time_passes(args.steps)  # synchronises the engine's stream
+    wall = time.perf_counter() - t0
+    tallies = eng.collect_tallies()
+    eng.close()
+
+    trials = args.steps * n_bp * 64
+    gate_evals = pd.n_gates * trials
+    value = gate_evals / wall
+    ab = algorithmic_bytes(pd, n_bp)
+    samp_s = dev["sampler_ms"] / 1e3 / args.steps
+    gate_s = dev["gates_ms"] / 1e3 / args.steps
+    dominant = "sample_events" if samp_s >= gate_s else "eval_gates"
+    if dominant == "sample_events":
+        achieved = ab["sampler"] / samp_s / 1e9
+        traffic_alg = ab["sampler"]
+        launches = 1
+    else:
+        achieved = ab["gates"] / gate_s / 1e9
+        traffic_alg = ab["gates"]
+        launches = info["num_layers"] - 1
+    pass_gbs = ab["pass"] / (wall / args.steps) / 1e9
+
+    out = {
+        "metric": "bit-level gate-evals/s",
+        "value": value,
+        "unit": "gate-evals/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u64",
+        "data": "synthetic",
+        "config": {"workload": f"plain MC, {args.tree} layered tree, every gate tallied",
+                   "basic_events": pd.n_vars, "gates": pd.n_gates, "gate_inputs": pd.n_args,
+                   "layers": info["num_layers"], "words_per_pass": n_bp,
+                   "trials_per_pass": n_bp * 64, "trials_timed": trials},
+        "node_evals_per_s": (pd.n_vars + pd.n_gates) * trials / wall,
+        "roofline": {"kernel": dominant, "bound": "hbm", "achieved": achieved,
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": None, "algorithmic_bytes_per_pass": traffic_alg,
+                     "launches_per_pass": launches},
+        "pass_roofline": {"bound": "hbm", "algorithmic_bytes_per_pass": ab["pass"],
+                          "achieved": pass_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                          "frac": pass_gbs / HBM_PEAK_GBS},
+        "device_ms_per_pass": {"sampler": samp_s * 1e3, "gates": gate_s * 1e3,
+                               "pass": dev["pass_ms"] / args.steps},
+        "root_estimate": next(t.mean for t in tallies if t.index == pd.root),
+    }
+
+    if not args.no_cpu:
+        words, ones, dt, threads = cpu_baseline(pd, args.cpu_seconds, 1 << 15)
+        with mc.LayerManager(pd, words, tally=tally) as small:
+            small.pass_(1)
+            gpu_ones = [t.num_one_bits for t in small.collect_tallies()]
+        out["cpu_baseline"] = {
+            "value": pd.n_gates * words * 64 / dt, "unit": "gate-evals/s", "cores": threads,
+            "kind": "port",
+            "sample": f"oracle, same tree, one pass of {words} words ({words * 64} trials), "
+                      f"every gate tallied, {dt:.1f} s on {threads} OpenMP threads",
+            "parity_bit_exact": [int(x) for x in ones] == gpu_ones}
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,236 @@
+/*
+ * mcscram_amd — MI355X-native Monte Carlo fault-tree engine, C ABI.
+ *
+ * Drop-in for the DirectEval Monte Carlo path of mcSCRAM (src/mc): the same
+ * PDAG input, the same probability-analysis settings (trials, burn-in, sample
+ * shape = batch_size x bitpacks_per_batch), the same Philox4x32-10 counter
+ * layout and threshold conversion, and bit-identical per-node tallies.
+ *
+ * Plain C types only; every call returns an mcs_status and leaves a message
+ * for mcs_last_error() on failure (the reference throws or exits instead:
+ * src/mc/queue/layer_manager.tpp:43,55,245; kernel_builder.h:341,359,713).
+ * One engine owns its device memory and one HIP stream. Calls on the same
+ * engine are not re-entrant; distinct engines are independent.
+ */
+#ifndef MCSCRAM_AMD_H
+#define MCSCRAM_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef int32_t mcs_status;
+#define MCS_OK 0
+#define MCS_ERR_INVALID_ARGUMENT 1 /* null pointer, bad size, p outside [0,1] */
+#define MCS_ERR_GRAPH 2            /* duplicate/unknown index, root not a gate, cycle */
+#define MCS_ERR_DEVICE 3           /* HIP runtime error */
+#define MCS_ERR_OUT_OF_MEMORY 4    /* device allocation would not fit */
+#define MCS_ERR_STATE 5            /* call not valid in the engine's state */
+
+/* Connective numbering of the reference PDAG (src/pdag.h:265-275). */
+enum mcs_connective {
+    MCS_AND = 0, MCS_OR = 1, MCS_ATLEAST = 2, MCS_XOR = 3,
+    MCS_NOT = 4, MCS_NAND = 5, MCS_NOR = 6, MCS_NULL = 7
+};
+
+/*
+ * Flattened PDAG (replaces core::Pdag as the engine's input, src/pdag.h:298-397).
+ * Variables: PDAG index (>= 2 in SCRAM, src/pdag.h:804) and probability.
+ * Gates: PDAG index, connective, min number (ATLEAST only; stored as 8 bits
+ * as the reference does, src/mc/event/node.h:676), and signed argument
+ * indices (negative = complement) in insertion order: gate arguments are
+ * visited in the order they appear in this list, as are variable arguments,
+ * which is the order of the reference's two argument maps
+ * (src/pdag.h:311-321). Arrays are copied at create time.
+ */
+typedef struct mcs_pdag {
+    int32_t n_vars;
+    const int32_t *var_index;   /* [n_vars] */
+    const double *var_p;        /* [n_vars], each in [0, 1] */
+    int32_t n_gates;
+    const int32_t *gate_index;  /* [n_gates] */
+    const uint8_t *gate_type;   /* [n_gates], enum mcs_connective */
+    const int32_t *gate_k;      /* [n_gates] */
+    const int64_t *arg_offsets; /* [n_gates + 1], CSR offsets into args */
+    const int32_t *args;        /* [arg_offsets[n_gates]] */
+    int32_t root;               /* PDAG index of the top gate */
+} mcs_pdag;
+
+/*
+ * Sample shape (src/mc/event/sample_shape.h:103-130): one pass evaluates
+ * batch_size * bitpacks_per_batch uint64 words = 64x that many trials per
+ * node. The reference derives it from device limits
+ * (src/mc/queue/sample_shaper.h:59-147); here it is an explicit input.
+ */
+typedef struct mcs_sample_shape {
+    uint64_t batch_size;
+    uint64_t bitpacks_per_batch;
+} mcs_sample_shape;
+
+/* Philox key. The reference hard-codes (382307844, 293830103)
+ * (src/mc/prng/philox128.h:39-42); any other key is a non-reference mode. */
+#define MCS_REFERENCE_KEY0 382307844u
+#define MCS_REFERENCE_KEY1 293830103u
+
+typedef struct mcs_engine mcs_engine;
+
+/* Per-node tally in the reference's record layout (src/mc/stats/tally.h:18-33):
+ * ci = {lower95, upper95, lower99, upper99}. */
+typedef struct mcs_tally {
+    int32_t index;
+    uint64_t num_one_bits;
+    uint64_t total_bits;
+    double mean;
+    double std_err;
+    double ci[4];
+} mcs_tally;
+
+typedef struct mcs_engine_info {
+    uint64_t num_bitpacks;     /* words per node per pass */
+    uint64_t row_stride;       /* words between node rows in device memory */
+    uint32_t iteration;        /* passes submitted so far (reference counter) */
+    int32_t num_events;        /* basic events reachable from the root */
+    int32_t num_gates;         /* gates reachable from the root */
+    int32_t num_layers;        /* layers including layer 0 (basic events) */
+    int32_t num_tallies;
+    uint64_t num_args;         /* total gate inputs (sum of fan-in) */
+    uint64_t device_bytes;     /* device memory owned by the engine */
+} mcs_engine_info;
+
+/* Message of the last failed call on this thread ("" if none). */
+const char *mcs_last_error(void);
+
+/*
+ * Build an engine: layered topological order, event positions, device
+ * blocks and tallies. Replaces
+ *   queue::layer_manager(core::Pdag*, num_trials, TallyNodeMap, overhead)
+ *   (src/mc/queue/layer_manager.h:284, layer_manager.tpp:344-375)
+ * with the sample shape given explicitly. tally_index lists the PDAG
+ * indices to tally (the reference's TallyNodeMap keys); duplicates collapse.
+ */
+mcs_status mcs_engine_create(const mcs_pdag *pdag, const mcs_sample_shape *shape,
+                             const int32_t *tally_index, int32_t n_tally,
+                             mcs_engine **out);
+
+/* As mcs_engine_create with a non-reference Philox key and an explicit HIP
+ * device ordinal (-1 = current device). */
+mcs_status mcs_engine_create_ex(const mcs_pdag *pdag, const mcs_sample_shape *shape,
+                                const int32_t *tally_index, int32_t n_tally,
+                                uint32_t key0, uint32_t key1, int32_t device,
+                                mcs_engine **out);
+
+/* Release all device memory (replaces ~layer_manager, layer_manager.tpp:281-301). */
+mcs_status mcs_engine_destroy(mcs_engine *engine);
+
+/* Enqueue n_passes passes without waiting: each advances the 1-based
+ * iteration counter and runs sampler -> gate layers -> tallies
+ * (replaces layer_manager::pass / single_pass, layer_manager.tpp:202-216;
+ * queueable.h:725-731 for the counter). */
+mcs_status mcs_engine_pass(mcs_engine *engine, uint32_t n_passes);
+
+/* Block until the engine's stream is idle (the reference's wait_and_throw,
+ * layer_manager.tpp:259). */
+mcs_status mcs_engine_wait(mcs_engine *engine);
+
+/* Enqueue total_passes passes, waiting every passes_between_waits passes
+ * (0 = only at the end), then collect (layer_manager.tpp:225-267). */
+mcs_status mcs_engine_pass_wait_collect(mcs_engine *engine, uint32_t total_passes,
+                                        uint32_t passes_between_waits,
+                                        mcs_tally *out, int32_t n_out);
+
+/* Copy tallies out with host-side point estimates (collect_tallies and
+ * stats::populate_point_estimates: layer_manager.tpp:270-279,
+ * src/mc/stats/ci_utils.h:167-201). n_out must equal the number of distinct
+ * tallied nodes; order = first appearance in tally_index. Waits first. */
+mcs_status mcs_engine_collect(mcs_engine *engine, mcs_tally *out, int32_t n_out);
+
+/* Engine geometry and counters. */
+mcs_status mcs_engine_get_info(mcs_engine *engine, mcs_engine_info *out);
+
+/* PDAG index of each basic event in event_idx order (the Philox counter's
+ * second word is event_idx + 1; src/mc/kernel/basic_event.h:112-122).
+ * n_out must be >= num_events. */
+mcs_status mcs_engine_event_order(mcs_engine *engine, int32_t *pdag_index_out, int32_t n_out);
+
+/* Host-only: the basic events' PDAG indices in event_idx order for a PDAG,
+ * without a device (the same plan mcs_engine_create builds). *n_events
+ * receives the count; pdag_index_out may be null to query it. */
+mcs_status mcs_pdag_event_order(const mcs_pdag *pdag, int32_t *pdag_index_out, int32_t n_out,
+                                int32_t *n_events);
+
+/* Host-only: depth (layer) of each gate slot of the input arrays, -1 for
+ * gates not reachable from the root. */
+mcs_status mcs_pdag_gate_depths(const mcs_pdag *pdag, int32_t *depth_out, int32_t n_out);
+
+/* Copy the words of one node's row from the last pass to host memory
+ * (num_bitpacks words). For parity checks and debugging. */
+mcs_status mcs_engine_read_words(mcs_engine *engine, int32_t pdag_index, uint64_t *out,
+                                 uint64_t n_out);
+
+/* Run n_passes passes with HIP events recorded on the engine's stream around
+ * each pass's sampler launch and around its gate launches, then wait.
+ * sampler_ms / gates_ms receive the summed device time of those launches
+ * over all passes, pass_ms the device time from the first pass's start to
+ * the last pass's end. Measurement hook; it has no counterpart in the
+ * reference (whose throughput line is host wall-clock,
+ * src/mc/scheduler/progressbar.h:525-645). */
+mcs_status mcs_engine_time_passes(mcs_engine *engine, uint32_t n_passes, double *sampler_ms,
+                                  double *gates_ms, double *pass_ms);
+
+/* Checkpoint / resume: set the iteration counter and the raw one-bit counts
+ * (order as in collect). The Philox counters are a pure function of
+ * (iteration, word), so resuming at iteration i continues the same stream. */
+mcs_status mcs_engine_restore(mcs_engine *engine, uint32_t iteration,
+                              const uint64_t *num_one_bits, int32_t n);
+
+/*
+ * Probability-analysis settings used by the DirectEval entry point
+ * (src/settings.h:439-464 defaults; src/mc/scheduler/convergence_controller.h).
+ */
+typedef struct mcs_settings {
+    uint64_t num_trials;       /* 0 = run until the early-stop rule (auto) */
+    uint64_t burn_in_trials;   /* reference default 2^20 */
+    double ci_confidence;      /* reference default 0.99 */
+    double ci_rel_margin;      /* relative half-width target, default 0.001 */
+    uint32_t max_passes;       /* safety cap for auto mode (0 = 1<<20) */
+    uint32_t key0, key1;       /* Philox key (reference key by default) */
+} mcs_settings;
+
+/* Two-sided standard-normal quantile z with P(|Z| <= z) = confidence
+ * (Acklam's inverse normal; the reference's src/mc/stats/ci_utils.h:30-92). */
+double mcs_normal_quantile_two_sided(double confidence);
+
+/* Fill settings with the reference defaults. */
+void mcs_settings_default(mcs_settings *s);
+
+/* Reference trial rounding and pass count: trials rounded to a multiple of
+ * 64 (ties down), passes = ceil(trials / (64 * words per pass))
+ * (src/mc/queue/sample_shaper.h:73-83,139-146). */
+mcs_status mcs_plan_passes(uint64_t num_trials, const mcs_sample_shape *shape,
+                           uint64_t *rounded_trials, uint64_t *passes);
+
+/*
+ * Entry point of the path: build the engine, run burn-in and then passes
+ * until the fixed trial count is reached (or, with num_trials == 0, until
+ * the root's CI half-width meets ci_rel_margin), collect tallies and return
+ * the root's mean. Replaces ProbabilityAnalyzer<mc::DirectEval>::
+ * CalculateTotalProbability / ComputeTallies
+ * (src/mc/core/probability_analysis.cc:88-209) and
+ * convergence_controller::run_to_convergence (convergence_controller.h:371-397).
+ * observe lists extra PDAG indices to tally besides the root; out receives
+ * root first, then the observed nodes (n_out >= 1 + distinct observed).
+ */
+mcs_status mcs_probability_analysis(const mcs_pdag *pdag, const mcs_settings *settings,
+                                    const mcs_sample_shape *shape,
+                                    const int32_t *observe, int32_t n_observe,
+                                    mcs_tally *out, int32_t n_out, double *p_total,
+                                    uint32_t *passes_run);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MCSCRAM_AMD_H */

@@ -1,0 +1,213 @@
+// Layered topological order and event positions, as the reference's
+// layer_manager computes them (see layering.h for the file:line map).
+#include "layering.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdlib>
+#include <unordered_map>
+
+namespace mcs {
+namespace {
+
+struct NodeRef {
+    bool is_var;
+    int32_t slot;
+};
+
+// One element of the sequence the reference sorts (its Node pointers).
+struct SortItem {
+    int32_t depth;
+    bool is_var;
+    uint8_t type;
+    int32_t slot;
+};
+
+}  // namespace
+
+bool build_plan(const mcs_pdag &pd, Plan &plan, std::string &error) {
+    if (pd.n_vars < 0 || pd.n_gates <= 0) {
+        error = "a PDAG needs at least one gate and a non-negative variable count";
+        return false;
+    }
+    if ((pd.n_vars && (!pd.var_index || !pd.var_p)) || !pd.gate_index || !pd.gate_type ||
+        !pd.gate_k || !pd.arg_offsets || (pd.arg_offsets[pd.n_gates] && !pd.args)) {
+        error = "null PDAG array";
+        return false;
+    }
+    std::unordered_map<int32_t, NodeRef> by_index;
+    by_index.reserve(size_t(pd.n_vars) + size_t(pd.n_gates));
+    for (int32_t i = 0; i < pd.n_vars; ++i) {
+        const double p = pd.var_p[i];
+        if (!(p >= 0.0 && p <= 1.0)) {
+            error = "probability of variable " + std::to_string(pd.var_index[i]) + " outside [0, 1]";
+            return false;
+        }
+        if (pd.var_index[i] <= 0 || !by_index.emplace(pd.var_index[i], NodeRef{true, i}).second) {
+            error = "duplicate or non-positive variable index " + std::to_string(pd.var_index[i]);
+            return false;
+        }
+    }
+    for (int32_t i = 0; i < pd.n_gates; ++i) {
+        if (pd.gate_type[i] > MCS_NULL) {
+            error = "unknown connective for gate " + std::to_string(pd.gate_index[i]);
+            return false;
+        }
+        if (pd.gate_index[i] <= 0 || !by_index.emplace(pd.gate_index[i], NodeRef{false, i}).second) {
+            error = "duplicate or non-positive gate index " + std::to_string(pd.gate_index[i]);
+            return false;
+        }
+        if (pd.arg_offsets[i + 1] < pd.arg_offsets[i]) {
+            error = "argument offsets are not monotone";
+            return false;
+        }
+        if (pd.arg_offsets[i + 1] - pd.arg_offsets[i] > 0xFFFF) {
+            error = "gate " + std::to_string(pd.gate_index[i]) + " has more than 65535 arguments";
+            return false;
+        }
+    }
+    if (pd.arg_offsets[0] != 0) {
+        error = "argument offsets must start at 0";
+        return false;
+    }
+    std::vector<std::vector<NodeRef>> args(pd.n_gates);
+    for (int32_t i = 0; i < pd.n_gates; ++i) {
+        for (int64_t a = pd.arg_offsets[i]; a < pd.arg_offsets[i + 1]; ++a) {
+            const int32_t s = pd.args[a];
+            auto it = by_index.find(s < 0 ? -s : s);
+            if (s == 0 || it == by_index.end()) {
+                error = "gate " + std::to_string(pd.gate_index[i]) + " has unknown argument " +
+                        std::to_string(s);
+                return false;
+            }
+            args[i].push_back(it->second);
+        }
+    }
+    auto root_it = by_index.find(pd.root);
+    if (root_it == by_index.end() || root_it->second.is_var) {
+        error = "root " + std::to_string(pd.root) + " is not a gate";
+        return false;
+    }
+    const int32_t root = root_it->second.slot;
+
+    // Depth of every gate reachable from the root (iterative post-order with
+    // cycle detection).
+    std::vector<int32_t> depth(pd.n_gates, -1);
+    std::vector<uint8_t> state(pd.n_gates, 0);  // 0 new, 1 open, 2 done
+    {
+        std::vector<std::pair<int32_t, size_t>> stack;
+        stack.push_back({root, 0});
+        state[root] = 1;
+        while (!stack.empty()) {
+            auto &[g, pos] = stack.back();
+            if (pos < args[g].size()) {
+                const NodeRef r = args[g][pos++];
+                if (r.is_var) continue;
+                if (state[r.slot] == 1) {
+                    error = "the PDAG has a cycle through gate " + std::to_string(pd.gate_index[r.slot]);
+                    return false;
+                }
+                if (state[r.slot] == 0) {
+                    state[r.slot] = 1;
+                    stack.push_back({r.slot, 0});
+                }
+                continue;
+            }
+            int32_t m = -1;
+            for (const NodeRef &r : args[g]) m = std::max(m, r.is_var ? 0 : depth[r.slot]);
+            depth[g] = m + 1;
+            state[g] = 2;
+            stack.pop_back();
+        }
+    }
+
+    // DFS gather, in the order of the reference's recursion: the gate, its
+    // gate arguments recursively, then its not yet visited variables.
+    std::vector<SortItem> seq;
+    {
+        std::vector<uint8_t> gvis(pd.n_gates, 0), vvis(pd.n_vars, 0);
+        std::vector<std::pair<int32_t, size_t>> stack;
+        gvis[root] = 1;
+        seq.push_back({depth[root], false, pd.gate_type[root], root});
+        stack.push_back({root, 0});
+        while (!stack.empty()) {
+            auto &[g, pos] = stack.back();
+            bool descended = false;
+            while (pos < args[g].size()) {
+                const NodeRef r = args[g][pos++];
+                if (r.is_var || gvis[r.slot]) continue;
+                gvis[r.slot] = 1;
+                seq.push_back({depth[r.slot], false, pd.gate_type[r.slot], r.slot});
+                stack.push_back({r.slot, 0});
+                descended = true;
+                break;
+            }
+            if (descended) continue;
+            for (const NodeRef &r : args[g]) {
+                if (r.is_var && !vvis[r.slot]) {
+                    vvis[r.slot] = 1;
+                    seq.push_back({0, true, 0, r.slot});
+                }
+            }
+            stack.pop_back();
+        }
+    }
+
+    // The reference's two unstable sorts, with its comparators.
+    std::sort(seq.begin(), seq.end(),
+              [](const SortItem &a, const SortItem &b) { return a.depth < b.depth; });
+    std::vector<std::vector<SortItem>> layers(size_t(seq.back().depth) + 1);
+    for (const SortItem &n : seq) layers[n.depth].push_back(n);
+    for (auto &layer : layers) {
+        std::sort(layer.begin(), layer.end(), [](const SortItem &l, const SortItem &r) {
+            if (l.is_var && !r.is_var) return true;
+            if (!l.is_var && r.is_var) return false;
+            if (l.is_var && r.is_var) return false;
+            return l.type < r.type;
+        });
+    }
+
+    plan = Plan{};
+    std::vector<int32_t> node_of_var(pd.n_vars, -1), node_of_gate(pd.n_gates, -1);
+    for (const SortItem &n : layers[0]) {
+        if (!n.is_var) {
+            error = "internal: gate at depth 0";
+            return false;
+        }
+        node_of_var[n.slot] = plan.n_events();
+        plan.event_pdag_index.push_back(pd.var_index[n.slot]);
+        plan.event_p.push_back(pd.var_p[n.slot]);
+    }
+    const int32_t n_events = plan.n_events();
+    int32_t next = n_events;
+    plan.layer_begin.push_back(0);
+    for (size_t l = 1; l < layers.size(); ++l) {
+        for (const SortItem &n : layers[l]) node_of_gate[n.slot] = next++;
+        plan.layer_begin.push_back(uint32_t(next - n_events));
+    }
+    plan.gates.reserve(size_t(next - n_events));
+    for (size_t l = 1; l < layers.size(); ++l) {
+        for (const SortItem &n : layers[l]) {
+            PlannedGate pg;
+            pg.pdag_index = pd.gate_index[n.slot];
+            pg.type = pd.gate_type[n.slot];
+            pg.k = pd.gate_k[n.slot];
+            pg.layer = int32_t(l);
+            for (int kind = 0; kind < 2; ++kind) {  // variables first, then gates
+                for (int64_t a = pd.arg_offsets[n.slot]; a < pd.arg_offsets[n.slot + 1]; ++a) {
+                    const NodeRef r = args[n.slot][size_t(a - pd.arg_offsets[n.slot])];
+                    if (r.is_var != (kind == 0)) continue;
+                    const int32_t id = r.is_var ? node_of_var[r.slot] : node_of_gate[r.slot];
+                    (pd.args[a] < 0 ? pg.neg_in : pg.pos_in).push_back(id);
+                }
+            }
+            plan.gates.push_back(std::move(pg));
+        }
+    }
+    plan.node_pdag_index = plan.event_pdag_index;
+    for (const PlannedGate &g : plan.gates) plan.node_pdag_index.push_back(g.pdag_index);
+    plan.root_node = node_of_gate[root];
+    return true;
+}
+
+}  // namespace mcs

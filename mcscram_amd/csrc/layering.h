@@ -1,0 +1,48 @@
+// Host-side graph compilation: flattened PDAG -> layered evaluation plan.
+#pragma once
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "mcscram_amd.h"
+
+namespace mcs {
+
+struct PlannedGate {
+    int32_t pdag_index;
+    uint8_t type;
+    int32_t k;
+    int32_t layer;                 // node depth (>= 1)
+    std::vector<int32_t> pos_in;   // input node ids (see Plan::node_of_*)
+    std::vector<int32_t> neg_in;
+};
+
+// Evaluation plan. Node ids: basic events are 0..n_events-1 in event_idx
+// order; gates follow in evaluation order (layer by layer).
+struct Plan {
+    std::vector<int32_t> event_pdag_index;  // by event_idx
+    std::vector<double> event_p;            // by event_idx
+    std::vector<PlannedGate> gates;         // evaluation order
+    std::vector<uint32_t> layer_begin;      // gate offsets per layer (layers 1..L), size L+1
+    std::vector<int32_t> node_pdag_index;   // by node id
+    int32_t root_node = -1;
+
+    int32_t n_events() const { return int32_t(event_pdag_index.size()); }
+    int32_t n_gates() const { return int32_t(gates.size()); }
+    int32_t n_nodes() const { return n_events() + n_gates(); }
+    int32_t n_gate_layers() const { return int32_t(layer_begin.size()) - 1; }
+};
+
+// Builds the plan the reference's layer_manager builds:
+//  * depth per node (variables 0, gate = 1 + max child depth;
+//    src/preprocessor.cc:130-176);
+//  * the DFS gather from the root (layer_manager.tpp:33-60) followed by the
+//    two unstable std::sort calls (tpp:79-82 by depth; tpp:92-115 variables
+//    first, gates by connective); event_idx = position in layer 0;
+//  * per-gate input order: positive then negated, variables before gates
+//    within each (kernel_builder.h:298-426).
+// Returns false with a message on malformed input.
+bool build_plan(const mcs_pdag &pdag, Plan &plan, std::string &error);
+
+}  // namespace mcs

@@ -1,0 +1,298 @@
+"""ctypes mirror of the engine's C ABI (include/mcscram_amd.h).
+
+Names follow the reference's Monte Carlo interface:
+``LayerManager`` is queue::layer_manager (src/mc/queue/layer_manager.h:284),
+``pass_wait_collect`` / ``collect_tallies`` its methods
+(layer_manager.tpp:225-279), and ``probability_analysis`` is
+ProbabilityAnalyzer<mc::DirectEval>::CalculateTotalProbability
+(src/mc/core/probability_analysis.cc:88-209). Errors raise ``EngineError``
+with the library's message.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+from .pdag import Pdag
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+MCS_REFERENCE_KEY = (382307844, 293830103)
+
+
+class EngineError(RuntimeError):
+    def __init__(self, status: int, message: str):
+        super().__init__(f"[status {status}] {message}")
+        self.status = status
+
+
+class _Pdag(C.Structure):
+    _fields_ = [("n_vars", C.c_int32), ("var_index", C.c_void_p), ("var_p", C.c_void_p),
+                ("n_gates", C.c_int32), ("gate_index", C.c_void_p), ("gate_type", C.c_void_p),
+                ("gate_k", C.c_void_p), ("arg_offsets", C.c_void_p), ("args", C.c_void_p),
+                ("root", C.c_int32)]
+
+
+class _Shape(C.Structure):
+    _fields_ = [("batch_size", C.c_uint64), ("bitpacks_per_batch", C.c_uint64)]
+
+
+class _Tally(C.Structure):
+    _fields_ = [("index", C.c_int32), ("num_one_bits", C.c_uint64), ("total_bits", C.c_uint64),
+                ("mean", C.c_double), ("std_err", C.c_double), ("ci", C.c_double * 4)]
+
+
+class _Info(C.Structure):
+    _fields_ = [("num_bitpacks", C.c_uint64), ("row_stride", C.c_uint64),
+                ("iteration", C.c_uint32), ("num_events", C.c_int32), ("num_gates", C.c_int32),
+                ("num_layers", C.c_int32), ("num_tallies", C.c_int32), ("num_args", C.c_uint64),
+                ("device_bytes", C.c_uint64)]
+
+
+class _Settings(C.Structure):
+    _fields_ = [("num_trials", C.c_uint64), ("burn_in_trials", C.c_uint64),
+                ("ci_confidence", C.c_double), ("ci_rel_margin", C.c_double),
+                ("max_passes", C.c_uint32), ("key0", C.c_uint32), ("key1", C.c_uint32)]
+
+
+def library_path() -> str:
+    return os.path.join(_HERE, "libmcscram_amd.so")
+
+
+def load_library():
+    """Load libmcscram_amd.so; raise if it has not been built."""
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    path = library_path()
+    if not os.path.exists(path):
+        raise RuntimeError(f"{path} is missing: build it with `python -m mcscram_amd.build`")
+    L = C.CDLL(path)
+    vp, i32, u32, u64 = C.c_void_p, C.c_int32, C.c_uint32, C.c_uint64
+    L.mcs_last_error.restype = C.c_char_p
+    sigs = {
+        "mcs_engine_create": [vp, vp, vp, i32, vp],
+        "mcs_engine_create_ex": [vp, vp, vp, i32, u32, u32, i32, vp],
+        "mcs_engine_destroy": [vp],
+        "mcs_engine_pass": [vp, u32],
+        "mcs_engine_wait": [vp],
+        "mcs_engine_pass_wait_collect": [vp, u32, u32, vp, i32],
+        "mcs_engine_collect": [vp, vp, i32],
+        "mcs_engine_get_info": [vp, vp],
+        "mcs_engine_event_order": [vp, vp, i32],
+        "mcs_engine_read_words": [vp, i32, vp, u64],
+        "mcs_engine_restore": [vp, u32, vp, i32],
+        "mcs_engine_time_passes": [vp, u32, vp, vp, vp],
+        "mcs_plan_passes": [u64, vp, vp, vp],
+        "mcs_pdag_event_order": [vp, vp, i32, vp],
+        "mcs_pdag_gate_depths": [vp, vp, i32],
+        "mcs_probability_analysis": [vp, vp, vp, vp, i32, vp, i32, vp, vp],
+    }
+    for name, args in sigs.items():
+        fn = getattr(L, name)
+        fn.argtypes = args
+        fn.restype = i32
+    L.mcs_settings_default.argtypes = [vp]
+    L.mcs_settings_default.restype = None
+    L.mcs_normal_quantile_two_sided.argtypes = [C.c_double]
+    L.mcs_normal_quantile_two_sided.restype = C.c_double
+    _LIB = L
+    return L
+
+
+def _check(status: int) -> None:
+    if status != 0:
+        raise EngineError(status, load_library().mcs_last_error().decode())
+
+
+def _p(a: np.ndarray):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+class _PdagView:
+    """Keeps contiguous copies alive while the C struct points at them."""
+
+    def __init__(self, pd: Pdag):
+        self.arrays = [np.ascontiguousarray(pd.var_index, np.int32),
+                       np.ascontiguousarray(pd.var_p, np.float64),
+                       np.ascontiguousarray(pd.gate_index, np.int32),
+                       np.ascontiguousarray(pd.gate_type, np.uint8),
+                       np.ascontiguousarray(pd.gate_k, np.int32),
+                       np.ascontiguousarray(pd.arg_off, np.int64),
+                       np.ascontiguousarray(pd.args, np.int32)]
+        a = self.arrays
+        self.struct = _Pdag(a[0].shape[0], _p(a[0]), _p(a[1]), a[2].shape[0], _p(a[2]),
+                            _p(a[3]), _p(a[4]), _p(a[5]), _p(a[6]), int(pd.root))
+
+
+@dataclass
+class Tally:
+    """Reference tally record (src/mc/stats/tally.h:18-33); ci = (lo95, hi95, lo99, hi99)."""
+
+    index: int
+    num_one_bits: int
+    total_bits: int
+    mean: float
+    std_err: float
+    ci: tuple
+
+    @staticmethod
+    def _from(t: _Tally) -> "Tally":
+        return Tally(t.index, int(t.num_one_bits), int(t.total_bits), t.mean, t.std_err, tuple(t.ci))
+
+
+def _shape(n_bp: int | tuple) -> _Shape:
+    if isinstance(n_bp, tuple):
+        return _Shape(int(n_bp[0]), int(n_bp[1]))
+    return _Shape(1, int(n_bp))
+
+
+class LayerManager:
+    """One compiled PDAG on one GPU (queue::layer_manager).
+
+    ``shape`` is the words per pass (int) or (batch_size, bitpacks_per_batch).
+    ``tally`` lists PDAG indices to tally (default: the root).
+    """
+
+    def __init__(self, pdag: Pdag, shape, tally=None, key=MCS_REFERENCE_KEY, device: int = -1):
+        L = load_library()
+        self._view = _PdagView(pdag)
+        self._shape = _shape(shape)
+        idx = np.ascontiguousarray([pdag.root] if tally is None else tally, dtype=np.int32)
+        handle = C.c_void_p()
+        _check(L.mcs_engine_create_ex(C.byref(self._view.struct), C.byref(self._shape), _p(idx),
+                                      idx.shape[0], key[0] & 0xFFFFFFFF, key[1] & 0xFFFFFFFF,
+                                      device, C.byref(handle)))
+        self._h = handle
+        self.info = self.get_info()
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            load_library().mcs_engine_destroy(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def get_info(self) -> dict:
+        info = _Info()
+        _check(load_library().mcs_engine_get_info(self._h, C.byref(info)))
+        return {f: getattr(info, f) for f, _ in _Info._fields_}
+
+    def pass_(self, n: int = 1) -> None:
+        _check(load_library().mcs_engine_pass(self._h, n))
+
+    def wait(self) -> None:
+        _check(load_library().mcs_engine_wait(self._h))
+
+    def collect_tallies(self) -> list[Tally]:
+        n = self.get_info()["num_tallies"]
+        out = (_Tally * n)()
+        _check(load_library().mcs_engine_collect(self._h, out, n))
+        return [Tally._from(t) for t in out]
+
+    def pass_wait_collect(self, total_passes: int = 1, passes_between_waits: int = 0) -> list[Tally]:
+        n = self.get_info()["num_tallies"]
+        out = (_Tally * n)()
+        _check(load_library().mcs_engine_pass_wait_collect(self._h, total_passes,
+                                                           passes_between_waits, out, n))
+        return [Tally._from(t) for t in out]
+
+    def event_order(self) -> np.ndarray:
+        n = self.info["num_events"]
+        out = np.zeros(max(n, 1), dtype=np.int32)
+        _check(load_library().mcs_engine_event_order(self._h, _p(out), out.shape[0]))
+        return out[:n]
+
+    def read_words(self, pdag_index: int) -> np.ndarray:
+        n = self.info["num_bitpacks"]
+        out = np.zeros(n, dtype=np.uint64)
+        _check(load_library().mcs_engine_read_words(self._h, int(pdag_index), _p(out), n))
+        return out
+
+    def time_passes(self, n: int) -> dict:
+        """Run n passes with HIP events on the engine's stream; device ms."""
+        s, g, t = C.c_double(), C.c_double(), C.c_double()
+        _check(load_library().mcs_engine_time_passes(self._h, n, C.byref(s), C.byref(g),
+                                                     C.byref(t)))
+        return {"sampler_ms": s.value, "gates_ms": g.value, "pass_ms": t.value}
+
+    def restore(self, iteration: int, num_one_bits) -> None:
+        ones = np.ascontiguousarray(num_one_bits, dtype=np.uint64)
+        _check(load_library().mcs_engine_restore(self._h, iteration, _p(ones), ones.shape[0]))
+
+
+@dataclass
+class Settings:
+    """Probability-analysis settings (src/settings.h:439-464 defaults)."""
+
+    num_trials: int = 0
+    burn_in_trials: int = 1 << 20
+    ci_confidence: float = 0.99
+    ci_rel_margin: float = 0.001
+    max_passes: int = 0
+    key: tuple = MCS_REFERENCE_KEY
+
+    def _c(self) -> _Settings:
+        return _Settings(self.num_trials, self.burn_in_trials, self.ci_confidence,
+                         self.ci_rel_margin, self.max_passes, self.key[0], self.key[1])
+
+
+def probability_analysis(pdag: Pdag, settings: Settings, shape, observe=()):
+    """-> (p_total, [root tally, observed tallies...], passes run)."""
+    L = load_library()
+    view = _PdagView(pdag)
+    sh = _shape(shape)
+    obs = np.ascontiguousarray(list(observe) or [0], dtype=np.int32)
+    n_obs = len(observe)
+    out = (_Tally * (1 + n_obs))()
+    p = C.c_double()
+    passes = C.c_uint32()
+    st = settings._c()
+    _check(L.mcs_probability_analysis(C.byref(view.struct), C.byref(st), C.byref(sh), _p(obs),
+                                      n_obs, out, 1 + n_obs, C.byref(p), C.byref(passes)))
+    distinct = 1 + len(set(observe) - {pdag.root})
+    return p.value, [Tally._from(t) for t in out[:distinct]], passes.value
+
+
+def plan_passes(num_trials: int, shape) -> tuple[int, int]:
+    sh = _shape(shape)
+    t, n = C.c_uint64(), C.c_uint64()
+    _check(load_library().mcs_plan_passes(num_trials, C.byref(sh), C.byref(t), C.byref(n)))
+    return t.value, n.value
+
+
+def pdag_event_order(pdag: Pdag) -> np.ndarray:
+    """Host-only: PDAG indices of the basic events in event_idx order."""
+    L = load_library()
+    view = _PdagView(pdag)
+    n = C.c_int32()
+    _check(L.mcs_pdag_event_order(C.byref(view.struct), None, 0, C.byref(n)))
+    out = np.zeros(max(n.value, 1), dtype=np.int32)
+    _check(L.mcs_pdag_event_order(C.byref(view.struct), _p(out), out.shape[0], C.byref(n)))
+    return out[:n.value]
+
+
+def pdag_gate_depths(pdag: Pdag) -> np.ndarray:
+    """Host-only: layer of each gate slot (-1 if unreachable)."""
+    view = _PdagView(pdag)
+    out = np.zeros(pdag.n_gates, dtype=np.int32)
+    _check(load_library().mcs_pdag_gate_depths(C.byref(view.struct), _p(out), out.shape[0]))
+    return out
+
+
+def normal_quantile_two_sided(confidence: float) -> float:
+    return load_library().mcs_normal_quantile_two_sided(confidence)

@@ -1,0 +1,117 @@
+"""The C ABI library loads, exports every entry point include/mcscram_amd.h
+declares, and its host-only logic (argument checks, pass planning, CI
+quantiles) behaves as the reference's. No device is touched here.
+"""
+import ctypes as C
+import os
+import re
+
+import numpy as np
+import pytest
+
+import mcscram_amd as mc
+from mcscram_amd.engine import EngineError, _PdagView, _Shape, load_library
+from mcscram_amd.pdag import AND, ATLEAST, OR, Pdag
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _declared():
+    names = set()
+    for f in os.listdir(os.path.join(ROOT, "include")):
+        if f.endswith(".h"):
+            text = open(os.path.join(ROOT, "include", f)).read()
+            names |= set(re.findall(r"\b(mcs_[a-z0-9_]+)\s*\(", text))
+    return sorted(names)
+
+
+def test_library_exports_every_declared_symbol():
+    lib = load_library()
+    declared = _declared()
+    assert len(declared) >= 15
+    for name in declared:
+        assert hasattr(lib, name), name
+
+
+def test_library_is_built_for_gfx950():
+    data = open(mc.library_path(), "rb").read()
+    assert b"gfx950" in data
+
+
+def test_plan_passes_matches_reference_rounding():
+    # trials -> nearest multiple of 64, ties down; passes = ceil(t / (64 N_bp))
+    # (src/mc/queue/sample_shaper.h:73-83, 139-146).
+    assert mc.plan_passes(100, 1) == (128, 2)
+    assert mc.plan_passes(96, 1) == (64, 1)    # tie rounds down
+    assert mc.plan_passes(97, 1) == (128, 2)
+    assert mc.plan_passes(1 << 20, (16, 1024)) == (1 << 20, 1)
+    assert mc.plan_passes(1 << 32, 1 << 18) == (1 << 32, 256)
+    assert mc.plan_passes(10, 4) == (0, 0)
+
+
+def test_normal_quantile_matches_scipy():
+    from scipy.stats import norm
+    # Acklam's approximation has relative error below 1.15e-9.
+    for c in (0.5, 0.9, 0.95, 0.99, 0.999, 0.999999):
+        z = norm.ppf(0.5 + c / 2)
+        assert abs(mc.normal_quantile_two_sided(c) - z) <= 1.2e-9 * z
+
+
+def _bad(pd, shape=(1, 64), tally=None, code=None):
+    L = load_library()
+    view = _PdagView(pd)
+    sh = _Shape(*shape)
+    idx = np.array(tally or [pd.root], dtype=np.int32)
+    h = C.c_void_p()
+    st = L.mcs_engine_create(C.byref(view.struct), C.byref(sh), idx.ctypes.data_as(C.c_void_p),
+                             idx.shape[0], C.byref(h))
+    assert st != 0
+    if code is not None:
+        assert st == code, L.mcs_last_error()
+    assert h.value is None
+    assert L.mcs_last_error()
+
+
+def test_create_rejects_bad_input_before_touching_a_device():
+    base = [(2, 0.1), (3, 0.2)]
+    _bad(Pdag.from_lists([(2, 1.5), (3, 0.2)], [(4, OR, 0, [2, 3])], 4), code=2)   # p > 1
+    _bad(Pdag.from_lists(base, [(4, OR, 0, [2, 9])], 4), code=2)                  # unknown arg
+    _bad(Pdag.from_lists(base, [(4, OR, 0, [2, 5]), (5, AND, 0, [3, 4])], 4), code=2)  # cycle
+    _bad(Pdag.from_lists(base, [(4, OR, 0, [2, 3])], 2), code=2)                  # root is a variable
+    _bad(Pdag.from_lists(base + [(4, 0.3)], [(4, OR, 0, [2, 3])], 4), code=2)     # duplicate index
+    _bad(Pdag.from_lists(base, [(4, OR, 0, [2, 3])], 4), shape=(0, 64), code=1)   # empty shape
+    _bad(Pdag.from_lists(base, [(4, OR, 0, [2, 3])], 4), tally=[99], code=2)      # unknown tally
+    _bad(Pdag.from_lists(base, [(4, 9, 0, [2, 3])], 4), code=2)                   # bad connective
+
+
+def test_null_handles():
+    L = load_library()
+    assert L.mcs_engine_pass(None, 1) == 1
+    assert L.mcs_engine_wait(None) == 1
+    assert L.mcs_engine_collect(None, None, 0) == 1
+    assert L.mcs_engine_destroy(None) == 0
+    assert L.mcs_probability_analysis(None, None, None, None, 0, None, 0, None, None) == 1
+
+
+def test_settings_defaults_are_the_reference_defaults():
+    s = mc.Settings()
+    assert (s.num_trials, s.burn_in_trials, s.ci_confidence, s.ci_rel_margin) == \
+        (0, 1 << 20, 0.99, 0.001)
+    L = load_library()
+    from mcscram_amd.engine import _Settings
+    cs = _Settings()
+    L.mcs_settings_default(C.byref(cs))
+    assert (cs.num_trials, cs.burn_in_trials, cs.ci_confidence, cs.ci_rel_margin) == \
+        (0, 1 << 20, 0.99, 0.001)
+    assert (cs.key0, cs.key1) == (382307844, 293830103)
+
+
+def test_generators_are_deterministic_and_sized():
+    a, b = mc.small_tree(1), mc.small_tree(1)
+    assert np.array_equal(a.args, b.args) and np.array_equal(a.var_p, b.var_p)
+    assert a.n_vars == 200 and a.n_gates == 120
+    assert np.all((a.var_p >= 1e-4) & (a.var_p <= 1e-1))
+    a.validate()
+    t = mc.pdag.Pdag  # noqa: F841
+    counts = np.bincount(a.gate_type, minlength=3)
+    assert counts[ATLEAST] > 0 and counts[AND] > 0 and counts[OR] > 0

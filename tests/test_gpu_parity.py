@@ -1,0 +1,163 @@
+"""GPU engine against the CPU oracle, through the C ABI: bit-exact words and
+tallies on the same inputs (seeded synthetic trees, the reference's fixed
+Philox key), plus size-independent properties at larger sizes.
+"""
+import numpy as np
+import pytest
+
+import mcscram_amd as mc
+from mcscram_amd.pdag import ATLEAST, OR, Pdag
+from oracle import binding as orc
+from tests.exact import exact_probabilities
+from tests.models import core_models, small_random
+
+pytestmark = pytest.mark.gpu
+
+
+def _rows(pd):
+    row = {int(v): s for s, v in enumerate(pd.var_index)}
+    row.update({int(g): pd.n_vars + s for s, g in enumerate(pd.gate_index)})
+    return row
+
+
+def _all_nodes(pd):
+    return [int(g) for g in pd.gate_index] + [int(v) for v in pd.var_index]
+
+
+def _engine_ones(eng):
+    return {t.index: t.num_one_bits for t in eng.collect_tallies()}
+
+
+def _check_tallies(pd, shape, n_pass=1, tally=None):
+    tally = tally or _all_nodes(pd)
+    n_bp = shape[0] * shape[1] if isinstance(shape, tuple) else shape
+    with mc.LayerManager(pd, shape, tally=tally) as eng:
+        eng.pass_(n_pass)
+        got = _engine_ones(eng)
+        totals = {t.total_bits for t in eng.collect_tallies()}
+    want, _ = orc.run(pd, n_bp, first_iter=1, n_iters=n_pass, tally=tally)
+    assert totals == {n_pass * n_bp * 64}
+    bad = [(i, got[i], int(w)) for i, w in zip(tally, want) if got[i] != int(w)]
+    assert not bad, bad[:10]
+
+
+def test_sampler_words_bit_exact_small_tree(gpu):
+    pd = mc.small_tree(1)
+    n_bp = 16 * 1024
+    _, words = orc.run(pd, n_bp, tally=[pd.root], want_words=True)
+    row = _rows(pd)
+    with mc.LayerManager(pd, (16, 1024), tally=[pd.root]) as eng:
+        eng.pass_(1)
+        for v in pd.var_index[::7]:
+            assert np.array_equal(eng.read_words(int(v)), words[row[int(v)]]), int(v)
+        for g in pd.gate_index[::5]:
+            assert np.array_equal(eng.read_words(int(g)), words[row[int(g)]]), int(g)
+
+
+def test_small_tree_all_tallies_bit_exact(gpu):
+    # 200 events, 120 gates, 2^20 trials = 16 batches x 1024 bitpacks.
+    _check_tallies(mc.small_tree(1), (16, 1024))
+
+
+@pytest.mark.parametrize("seed", [1, 2, 3])
+def test_all_connectives_and_negations(gpu, seed):
+    pd = small_random(seed, n_be=60, n_gates=80, n_layers=7)
+    _check_tallies(pd, 2048)
+
+
+def test_several_passes_use_successive_iterations(gpu):
+    _check_tallies(small_random(4, n_be=40, n_gates=50, n_layers=6), 1536, n_pass=3)
+
+
+@pytest.mark.parametrize("shape", [1, 3, (3, 333), 511, 513, 4097])
+def test_ragged_word_counts(gpu, shape):
+    _check_tallies(small_random(6, n_be=25, n_gates=30, n_layers=5), shape)
+
+
+def test_probability_zero_and_one(gpu):
+    pd = Pdag.from_lists([(2, 0.0), (3, 1.0), (4, 0.5)],
+                         [(5, OR, 0, [2, 4]), (6, ATLEAST, 2, [3, 4, 5]), (7, OR, 0, [-3, 6])], 7)
+    _check_tallies(pd, 1000)
+    with mc.LayerManager(pd, 1000, tally=[2, 3]) as eng:
+        eng.pass_(1)
+        assert np.all(eng.read_words(2) == 0)
+        assert np.all(eng.read_words(3) == np.uint64(2**64 - 1))
+
+
+def test_wide_atleast_gate(gpu):
+    # 300 inputs: the 8-bit counter of the reference wraps (gate.h:526-541).
+    n = 300
+    vs = [(2 + i, 0.9) for i in range(n)]
+    gates = [(2 + n, ATLEAST, 40, [2 + i for i in range(n)]),
+             (3 + n, ATLEAST, 200, [2 + i for i in range(n)]),
+             (4 + n, OR, 0, [2 + n, 3 + n])]
+    _check_tallies(Pdag.from_lists(vs, gates, 4 + n), 512)
+
+
+def test_large_tree_prefix_bit_exact(gpu):
+    # 25,000 events / 40,000 gates, 2^20-trial prefix (2^14 words), every gate.
+    pd = mc.large_tree()
+    _check_tallies(pd, 1 << 14, tally=[int(g) for g in pd.gate_index])
+
+
+@pytest.mark.parametrize("name", sorted(core_models()))
+def test_core_known_answers_on_gpu(gpu, name):
+    pd, exact = core_models()[name]
+    settings = mc.Settings(num_trials=1 << 22, burn_in_trials=0)
+    p, tallies, passes = mc.probability_analysis(pd, settings, 1 << 14)
+    assert passes == 4
+    t = tallies[0]
+    assert t.total_bits == 1 << 22
+    assert abs(p - exact) <= 3.2905 * t.std_err, (name, p, exact)
+    assert t.ci[2] <= p <= t.ci[3]
+
+
+def test_small_tree_against_enumeration_on_gpu(gpu):
+    pd = small_random(21, n_be=14, n_gates=16, n_layers=5)
+    exact = exact_probabilities(pd)
+    tally = _all_nodes(pd)
+    with mc.LayerManager(pd, 1 << 15, tally=tally) as eng:
+        eng.pass_(2)
+        for t in eng.collect_tallies():
+            assert abs(t.mean - exact[t.index]) <= 3.2905 * max(t.std_err, 1e-9), t
+
+
+def test_checkpoint_resume_is_bit_identical(gpu):
+    pd = small_random(9, n_be=30, n_gates=40, n_layers=6)
+    tally = _all_nodes(pd)
+    with mc.LayerManager(pd, 1024, tally=tally) as a:
+        a.pass_(3)
+        full = [t.num_one_bits for t in a.collect_tallies()]
+    with mc.LayerManager(pd, 1024, tally=tally) as b:
+        b.pass_(1)
+        first = [t.num_one_bits for t in b.collect_tallies()]
+    with mc.LayerManager(pd, 1024, tally=tally) as c:
+        c.restore(1, first)
+        c.pass_(2)
+        resumed = [t.num_one_bits for t in c.collect_tallies()]
+        assert c.get_info()["iteration"] == 3
+    assert resumed == full
+
+
+def test_pass_wait_collect_intervals(gpu):
+    pd = small_random(10, n_be=20, n_gates=20, n_layers=4)
+    with mc.LayerManager(pd, 700) as a, mc.LayerManager(pd, 700) as b:
+        ta = a.pass_wait_collect(5, 0)
+        tb = b.pass_wait_collect(5, 2)
+    assert ta == tb
+
+
+def test_probability_analysis_early_stop(gpu):
+    pd, exact = core_models()["abc"]
+    settings = mc.Settings(num_trials=0, burn_in_trials=1 << 16, ci_rel_margin=0.01)
+    p, tallies, passes = mc.probability_analysis(pd, settings, 1 << 12)
+    t = tallies[0]
+    half = 2.5758293035489004 * t.std_err
+    assert half <= 0.01 * p * 1.000001
+    assert abs(p - exact) <= 3.2905 * t.std_err
+
+
+def test_event_order_on_engine_matches_host_plan(gpu):
+    pd = mc.small_tree(1)
+    with mc.LayerManager(pd, 64) as eng:
+        assert np.array_equal(eng.event_order(), mc.pdag_event_order(pd))
