@@ -83,17 +83,12 @@ def main() -> int:
     gate_evals = pd.n_gates * trials
     value = gate_evals / wall
     ab = algorithmic_bytes(pd, n_bp)
-    samp_s = dev["sampler_ms"] / 1e3 / args.steps
-    gate_s = dev["gates_ms"] / 1e3 / args.steps
-    dominant = "sample_events" if samp_s >= gate_s else "eval_gates"
-    if dominant == "sample_events":
-        achieved = ab["sampler"] / samp_s / 1e9
-        traffic_alg = ab["sampler"]
-        launches = 1
-    else:
-        achieved = ab["gates"] / gate_s / 1e9
-        traffic_alg = ab["gates"]
-        launches = info["num_layers"] - 1
+    # pass_layer is the only kernel: every launch carries one gate layer and
+    # a slice of the next pass's sampling.
+    launch_s = dev["kernel_ms"] / 1e3 / dev["launches"]
+    launches_per_pass = dev["launches"] / args.steps
+    bytes_per_launch = (ab["sampler"] + ab["gates"]) / launches_per_pass
+    achieved = bytes_per_launch / launch_s / 1e9
     pass_gbs = ab["pass"] / (wall / args.steps) / 1e9
 
     out = {
@@ -114,15 +109,16 @@ def main() -> int:
                    "layers": info["num_layers"], "words_per_pass": n_bp,
                    "trials_per_pass": n_bp * 64, "trials_timed": trials},
         "node_evals_per_s": (pd.n_vars + pd.n_gates) * trials / wall,
-        "roofline": {"kernel": dominant, "bound": "hbm", "achieved": achieved,
+        "roofline": {"kernel": "pass_layer", "bound": "hbm", "achieved": achieved,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": None, "algorithmic_bytes_per_pass": traffic_alg,
-                     "launches_per_pass": launches},
+                     "traffic": None, "algorithmic_bytes_per_launch": bytes_per_launch,
+                     "avg_launch_ms": launch_s * 1e3, "launches_per_pass": launches_per_pass},
         "pass_roofline": {"bound": "hbm", "algorithmic_bytes_per_pass": ab["pass"],
                           "achieved": pass_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                           "frac": pass_gbs / HBM_PEAK_GBS},
-        "device_ms_per_pass": {"sampler": samp_s * 1e3, "gates": gate_s * 1e3,
-                               "pass": dev["pass_ms"] / args.steps},
+        "device_ms_per_pass": {"kernels": dev["kernel_ms"] / args.steps,
+                               "first_to_last": dev["pass_ms"] / args.steps},
+        "event_buffers": info["event_buffers"],
         "root_estimate": next(t.mean for t in tallies if t.index == pd.root),
     }
 

@@ -98,6 +98,8 @@ typedef struct mcs_engine_info {
     int32_t num_tallies;
     uint64_t num_args;         /* total gate inputs (sum of fan-in) */
     uint64_t device_bytes;     /* device memory owned by the engine */
+    int32_t event_buffers;     /* 2: the next pass is sampled during this one */
+    uint32_t launches_per_pass;/* kernel launches of a pass in steady state */
 } mcs_engine_info;
 
 /* Message of the last failed call on this thread ("" if none). */
@@ -170,15 +172,14 @@ mcs_status mcs_pdag_gate_depths(const mcs_pdag *pdag, int32_t *depth_out, int32_
 mcs_status mcs_engine_read_words(mcs_engine *engine, int32_t pdag_index, uint64_t *out,
                                  uint64_t n_out);
 
-/* Run n_passes passes with HIP events recorded on the engine's stream around
- * each pass's sampler launch and around its gate launches, then wait.
- * sampler_ms / gates_ms receive the summed device time of those launches
- * over all passes, pass_ms the device time from the first pass's start to
- * the last pass's end. Measurement hook; it has no counterpart in the
- * reference (whose throughput line is host wall-clock,
- * src/mc/scheduler/progressbar.h:525-645). */
-mcs_status mcs_engine_time_passes(mcs_engine *engine, uint32_t n_passes, double *sampler_ms,
-                                  double *gates_ms, double *pass_ms);
+/* Run n_passes passes with a HIP event recorded on the engine's stream
+ * before and after every kernel launch, then wait. kernel_ms receives the
+ * summed device time of the launches, launches their number, pass_ms the
+ * device time from the first launch's start to the last launch's end.
+ * Measurement hook; it has no counterpart in the reference (whose
+ * throughput line is host wall-clock, src/mc/scheduler/progressbar.h:525-645). */
+mcs_status mcs_engine_time_passes(mcs_engine *engine, uint32_t n_passes, double *kernel_ms,
+                                  uint32_t *launches, double *pass_ms);
 
 /* Checkpoint / resume: set the iteration counter and the raw one-bit counts
  * (order as in collect). The Philox counters are a pure function of

@@ -1,8 +1,15 @@
 // Engine: device memory, pass scheduling, tallies and the C ABI.
 //
 // One engine = one compiled PDAG on one device with one HIP stream. Device
-// memory: a words matrix (rows = nodes, see kernels.h), event and gate
-// descriptors, argument rows, and one uint64 one-bit counter per row.
+// memory: the words matrix (layout in kernels.h), event and gate
+// descriptors, the argument rows of each event buffer, and one uint64
+// one-bit counter per row.
+//
+// A pass is one launch of pass_layer per gate layer. With two event buffers
+// the launches of pass i also sample pass i + 1 into the other buffer, a
+// slice per launch in proportion to the layer's gate traffic, so the Philox
+// work (VALU-bound) runs beside the gate evaluation (HBM-bound). Tallies of
+// basic events are tally-only tasks in the first launch of their pass.
 #include <hip/hip_runtime_api.h>
 
 #include <algorithm>
@@ -45,19 +52,35 @@ struct mcs_engine {
     hipStream_t stream = nullptr;
     uint32_t key0 = MCS_REFERENCE_KEY0, key1 = MCS_REFERENCE_KEY1;
     uint32_t iteration = 0;  // passes submitted (reference iterable_queueable counter)
+    uint32_t primed = 0;     // iteration already sampled into its buffer (0 = none)
     mcs::Plan plan;
     mcs::LaunchGeometry geo{};
-    std::vector<int32_t> tally_nodes;  // node id per tally, first-appearance order
-    std::vector<int32_t> tally_index;  // pdag index per tally
+    uint32_t n_buf = 1;                     // event buffers
+    std::vector<uint32_t> launch_gate_begin;// per launch: first gate desc (size L+1)
+    std::vector<uint64_t> launch_samp_begin;// per launch: first sampler task (size L+1)
+    std::vector<int32_t> tally_nodes;       // node id per tally, first-appearance order
+    std::vector<int32_t> tally_index;       // pdag index per tally
     std::unordered_map<int32_t, int32_t> node_of_index;
     uint64_t n_args = 0;
+    uint64_t n_rows = 0;
+    uint64_t arg_span = 0;                  // entries per argument-row array
     uint64_t device_bytes = 0;
     // device
     uint64_t *d_words = nullptr;
     mcs::EventDesc *d_events = nullptr;
     mcs::GateDesc *d_gates = nullptr;
-    uint32_t *d_args = nullptr;
+    uint32_t *d_args = nullptr;             // n_buf arrays of arg_span entries
     unsigned long long *d_ones = nullptr;
+
+    uint64_t gate_row(int32_t node) const {
+        return uint64_t(n_buf) * plan.n_events() + uint64_t(node - plan.n_events());
+    }
+    // Counter slot of a tallied node: event id for events, row for gates.
+    uint64_t tally_slot(int32_t node) const {
+        return node < plan.n_events() ? uint64_t(node) : gate_row(node);
+    }
+    uint32_t parity(uint32_t it) const { return n_buf == 2 ? (it & 1u) : 0u; }
+    uint32_t n_launches() const { return uint32_t(launch_gate_begin.size() - 1); }
 
     ~mcs_engine() {
         if (stream) {
@@ -75,9 +98,9 @@ struct mcs_engine {
 
 namespace {
 
-// Words per workgroup task: 512 (256 lanes x 2 words) times a repeat count
-// that keeps each grid large enough to fill 256 CUs but amortises per-task
-// set-up on long rows.
+// Words per workgroup task: 512 (256 lanes x 2 words) times 1, 2 or 4,
+// keeping grids far above the 256 CUs while amortising per-task set-up on
+// long rows.
 uint32_t pick_chunk_words(uint64_t n_bp) {
     if (n_bp >= (uint64_t(1) << 18)) return 2048;
     if (n_bp >= (uint64_t(1) << 16)) return 1024;
@@ -102,6 +125,7 @@ mcs_status build_engine(const mcs_pdag *pdag, const mcs_sample_shape *shape,
     std::string err;
     if (!mcs::build_plan(*pdag, eng->plan, err)) return fail(MCS_ERR_GRAPH, err);
     const mcs::Plan &plan = eng->plan;
+    const int32_t n_ev = plan.n_events();
     for (int32_t n = 0; n < plan.n_nodes(); ++n) eng->node_of_index[plan.node_pdag_index[n]] = n;
 
     std::vector<uint8_t> tallied(plan.n_nodes(), 0);
@@ -122,17 +146,97 @@ mcs_status build_engine(const mcs_pdag *pdag, const mcs_sample_shape *shape,
     geo.n_chunks = uint32_t((n_bp + geo.chunk_words - 1) / geo.chunk_words);
     geo.stride = uint64_t(geo.n_chunks) * geo.chunk_words;
     geo.n_bp = n_bp;
-    uint32_t max_layer = 0;
-    for (int32_t l = 0; l < plan.n_gate_layers(); ++l)
-        max_layer = std::max(max_layer, plan.layer_begin[l + 1] - plan.layer_begin[l]);
-    const uint64_t max_rows = std::max<uint64_t>(uint64_t(plan.n_events()), max_layer);
-    if (max_rows * geo.n_chunks * 256ull >= (uint64_t(1) << 32))
-        return fail(MCS_ERR_INVALID_ARGUMENT, "grid too large: reduce words per pass");
     eng->geo = geo;
 
-    // Descriptors.
-    std::vector<mcs::EventDesc> ev(plan.n_events());
-    for (int32_t e = 0; e < plan.n_events(); ++e) {
+    // Device and memory budget: two event buffers when they fit.
+    eng->device = device;
+    if (device < 0) MCS_HIP(hipGetDevice(&eng->device));
+    MCS_HIP(hipSetDevice(eng->device));
+    size_t free_b = 0, total_b = 0;
+    MCS_HIP(hipMemGetInfo(&free_b, &total_b));
+    const uint64_t row_bytes = geo.stride * sizeof(uint64_t);
+    const uint64_t reserve = uint64_t(256) << 20;
+    const uint64_t one_buf = (uint64_t(plan.n_nodes()) * row_bytes);
+    const uint64_t two_buf = one_buf + uint64_t(n_ev) * row_bytes;
+    eng->n_buf = (n_ev > 0 && two_buf + reserve <= free_b) ? 2u : 1u;
+    eng->n_rows = uint64_t(eng->n_buf) * n_ev + plan.n_gates();
+
+    // Launch plan: one launch per gate layer; launch 0 also carries the
+    // tally-only tasks of tallied basic events.
+    std::vector<mcs::GateDesc> gd;
+    std::vector<uint32_t> args0, args1;  // argument rows for event buffer 0 / 1
+    std::vector<double> layer_cost;
+    auto arg_row = [&](int32_t node, uint32_t buf) -> uint32_t {
+        return node < n_ev ? uint32_t(node + int64_t(buf) * n_ev) : uint32_t(eng->gate_row(node));
+    };
+    eng->launch_gate_begin.push_back(0);
+    for (int32_t l = 0; l < plan.n_gate_layers(); ++l) {
+        double cost = 0;
+        auto add_gate = [&](uint32_t out_row, uint32_t type, int32_t k, uint32_t flags,
+                            const std::vector<int32_t> &pos, const std::vector<int32_t> &neg) {
+            mcs::GateDesc d{};
+            d.out_row = out_row;
+            d.arg_begin = uint32_t(args0.size());
+            const uint32_t n_in = uint32_t(pos.size() + neg.size());
+            d.counts = n_in | uint32_t(pos.size()) << 16;
+            d.type_k_flags = type | (uint32_t(uint8_t(k)) << 8) | flags;
+            for (const auto *v : {&pos, &neg})
+                for (int32_t i : *v) {
+                    args0.push_back(arg_row(i, 0));
+                    args1.push_back(arg_row(i, 1));
+                }
+            gd.push_back(d);
+            cost += n_in + ((flags & mcs::kGateNoStore) ? 0 : 1);
+        };
+        for (uint32_t g = plan.layer_begin[l]; g < plan.layer_begin[l + 1]; ++g) {
+            const mcs::PlannedGate &pg = plan.gates[g];
+            const int32_t node = n_ev + int32_t(g);
+            add_gate(uint32_t(eng->gate_row(node)), pg.type, pg.k,
+                     tallied[node] ? mcs::kGateTallied : 0u, pg.pos_in, pg.neg_in);
+        }
+        if (l == 0) {
+            for (int32_t e = 0; e < n_ev; ++e)
+                if (tallied[e])
+                    add_gate(uint32_t(e), MCS_NULL, 0, mcs::kGateTallied | mcs::kGateNoStore, {e}, {});
+        }
+        eng->launch_gate_begin.push_back(uint32_t(gd.size()));
+        layer_cost.push_back(cost);
+    }
+    eng->n_args = args0.size();
+    eng->arg_span = std::max<uint64_t>(args0.size(), 1);
+
+    // Sampler tasks of the next pass, split over the launches in proportion
+    // to each launch's gate traffic.
+    const uint64_t samp_total = uint64_t(n_ev) * geo.n_chunks;
+    double cost_sum = 0;
+    for (double c : layer_cost) cost_sum += c;
+    eng->launch_samp_begin.push_back(0);
+    double acc = 0;
+    for (size_t l = 0; l < layer_cost.size(); ++l) {
+        acc += layer_cost[l];
+        const uint64_t end = (l + 1 == layer_cost.size() || cost_sum <= 0)
+                                 ? samp_total
+                                 : uint64_t(double(samp_total) * (acc / cost_sum));
+        eng->launch_samp_begin.push_back(std::max(end, eng->launch_samp_begin.back()));
+    }
+    for (uint32_t l = 0; l < eng->n_launches(); ++l) {
+        mcs::PassLaunch pl{};
+        pl.n_gates = eng->launch_gate_begin[l + 1] - eng->launch_gate_begin[l];
+        pl.samp_begin = eng->launch_samp_begin[l];
+        pl.samp_end = eng->launch_samp_begin[l + 1];
+        if (mcs::pass_grid_blocks(pl, geo) * 256ull >= (uint64_t(1) << 32))
+            return fail(MCS_ERR_INVALID_ARGUMENT, "grid too large: reduce words per pass");
+    }
+    {
+        mcs::PassLaunch pl{};
+        pl.samp_end = samp_total;
+        if (mcs::pass_grid_blocks(pl, geo) * 256ull >= (uint64_t(1) << 32))
+            return fail(MCS_ERR_INVALID_ARGUMENT, "grid too large: reduce words per pass");
+    }
+
+    // Event descriptors.
+    std::vector<mcs::EventDesc> ev(n_ev);
+    for (int32_t e = 0; e < n_ev; ++e) {
         const double p = plan.event_p[e];
         const uint64_t thr = static_cast<uint64_t>(p * 4294967296.0);
         uint32_t mode = mcs::kEventDraw;
@@ -141,54 +245,39 @@ mcs_status build_engine(const mcs_pdag *pdag, const mcs_sample_shape *shape,
         ev[e].ctr0 = uint32_t(plan.event_pdag_index[e]) + 1u;
         ev[e].ctr1 = uint32_t(e) + 1u;
         ev[e].threshold = uint32_t(std::min<uint64_t>(thr, 0xFFFFFFFFull));
-        ev[e].mode_tally = mode | (uint32_t(tallied[e]) << 2);
+        ev[e].mode = mode;
     }
-    std::vector<mcs::GateDesc> gd(plan.n_gates());
-    std::vector<uint32_t> arg_rows;
-    for (int32_t g = 0; g < plan.n_gates(); ++g) {
-        const mcs::PlannedGate &pg = plan.gates[g];
-        const int32_t node = plan.n_events() + g;
-        gd[g].out_row = uint32_t(node);
-        gd[g].arg_begin = uint32_t(arg_rows.size());
-        const uint32_t n_in = uint32_t(pg.pos_in.size() + pg.neg_in.size());
-        gd[g].counts = n_in | uint32_t(pg.pos_in.size()) << 16;
-        gd[g].type_k_tally = uint32_t(pg.type) | (uint32_t(uint8_t(pg.k)) << 8) |
-                             (uint32_t(tallied[node]) << 16);
-        for (int32_t i : pg.pos_in) arg_rows.push_back(uint32_t(i));
-        for (int32_t i : pg.neg_in) arg_rows.push_back(uint32_t(i));
-    }
-    eng->n_args = arg_rows.size();
 
     // Device allocation.
-    eng->device = device;
-    if (device < 0) MCS_HIP(hipGetDevice(&eng->device));
-    MCS_HIP(hipSetDevice(eng->device));
     MCS_HIP(hipStreamCreateWithFlags(&eng->stream, hipStreamNonBlocking));
-    const uint64_t rows = uint64_t(plan.n_nodes());
-    const uint64_t word_bytes = rows * geo.stride * sizeof(uint64_t);
-    size_t free_b = 0, total_b = 0;
-    MCS_HIP(hipMemGetInfo(&free_b, &total_b));
+    const uint64_t word_bytes = eng->n_rows * row_bytes;
     const uint64_t need = word_bytes + ev.size() * sizeof(mcs::EventDesc) +
-                          gd.size() * sizeof(mcs::GateDesc) + arg_rows.size() * 4 + rows * 8;
-    if (need + (uint64_t(256) << 20) > free_b)
+                          gd.size() * sizeof(mcs::GateDesc) + eng->arg_span * 4 * eng->n_buf +
+                          eng->n_rows * 8;
+    if (need + reserve > free_b)
         return fail(MCS_ERR_OUT_OF_MEMORY,
                     "engine needs " + std::to_string(need >> 20) + " MiB of device memory, " +
                         std::to_string(free_b >> 20) + " MiB free: reduce words per pass");
     MCS_HIP(hipMalloc(&eng->d_words, word_bytes));
-    MCS_HIP(hipMalloc(&eng->d_ones, rows * sizeof(unsigned long long)));
+    MCS_HIP(hipMalloc(&eng->d_ones, eng->n_rows * sizeof(unsigned long long)));
     if (!ev.empty()) MCS_HIP(hipMalloc(&eng->d_events, ev.size() * sizeof(mcs::EventDesc)));
-    MCS_HIP(hipMalloc(&eng->d_gates, gd.size() * sizeof(mcs::GateDesc)));
-    MCS_HIP(hipMalloc(&eng->d_args, std::max<size_t>(arg_rows.size(), 1) * 4));
+    MCS_HIP(hipMalloc(&eng->d_gates, std::max<size_t>(gd.size(), 1) * sizeof(mcs::GateDesc)));
+    MCS_HIP(hipMalloc(&eng->d_args, eng->arg_span * 4 * eng->n_buf));
     eng->device_bytes = need;
     if (!ev.empty())
         MCS_HIP(hipMemcpyAsync(eng->d_events, ev.data(), ev.size() * sizeof(mcs::EventDesc),
                                hipMemcpyHostToDevice, eng->stream));
-    MCS_HIP(hipMemcpyAsync(eng->d_gates, gd.data(), gd.size() * sizeof(mcs::GateDesc),
-                           hipMemcpyHostToDevice, eng->stream));
-    if (!arg_rows.empty())
-        MCS_HIP(hipMemcpyAsync(eng->d_args, arg_rows.data(), arg_rows.size() * 4,
+    if (!gd.empty())
+        MCS_HIP(hipMemcpyAsync(eng->d_gates, gd.data(), gd.size() * sizeof(mcs::GateDesc),
                                hipMemcpyHostToDevice, eng->stream));
-    MCS_HIP(hipMemsetAsync(eng->d_ones, 0, rows * sizeof(unsigned long long), eng->stream));
+    if (!args0.empty()) {
+        MCS_HIP(hipMemcpyAsync(eng->d_args, args0.data(), args0.size() * 4, hipMemcpyHostToDevice,
+                               eng->stream));
+        if (eng->n_buf == 2)
+            MCS_HIP(hipMemcpyAsync(eng->d_args + eng->arg_span, args1.data(), args1.size() * 4,
+                                   hipMemcpyHostToDevice, eng->stream));
+    }
+    MCS_HIP(hipMemsetAsync(eng->d_ones, 0, eng->n_rows * sizeof(unsigned long long), eng->stream));
     MCS_HIP(hipStreamSynchronize(eng->stream));
     eng->key0 = key0;
     eng->key1 = key1;
@@ -196,19 +285,60 @@ mcs_status build_engine(const mcs_pdag *pdag, const mcs_sample_shape *shape,
     return MCS_OK;
 }
 
-mcs_status enqueue_passes(mcs_engine *e, uint32_t n) {
+mcs::PassLaunch base_launch(const mcs_engine *e) {
+    mcs::PassLaunch pl{};
+    pl.events = e->d_events;
+    pl.key0 = e->key0;
+    pl.key1 = e->key1;
+    return pl;
+}
+
+// Launch hook: called around every kernel launch (for timing); null = none.
+struct LaunchHook {
+    virtual hipError_t before() = 0;
+    virtual hipError_t after() = 0;
+    virtual ~LaunchHook() = default;
+};
+
+hipError_t launch(mcs_engine *e, const mcs::PassLaunch &pl, LaunchHook *hook) {
+    if (mcs::pass_grid_blocks(pl, e->geo) == 0) return hipSuccess;
+    hipError_t st = hook ? hook->before() : hipSuccess;
+    if (st == hipSuccess) st = mcs::launch_pass(e->stream, pl, e->d_words, e->d_ones, e->geo);
+    if (st == hipSuccess && hook) st = hook->after();
+    return st;
+}
+
+mcs_status enqueue_passes(mcs_engine *e, uint32_t n, LaunchHook *hook = nullptr) {
     MCS_HIP(hipSetDevice(e->device));
-    const mcs::Plan &plan = e->plan;
+    const uint32_t n_ev = uint32_t(e->plan.n_events());
+    const uint64_t samp_total = uint64_t(n_ev) * e->geo.n_chunks;
     for (uint32_t i = 0; i < n; ++i) {
         if (e->iteration == 0xFFFFFFFFu) return fail(MCS_ERR_STATE, "iteration counter exhausted");
-        const uint32_t iteration = ++e->iteration;
-        MCS_HIP(mcs::launch_sample_events(e->stream, e->d_events, e->d_words, e->d_ones, e->geo,
-                                          uint32_t(plan.n_events()), iteration, e->key0, e->key1));
-        for (int32_t l = 0; l < plan.n_gate_layers(); ++l) {
-            const uint32_t b = plan.layer_begin[l], c = plan.layer_begin[l + 1] - b;
-            MCS_HIP(mcs::launch_eval_gates(e->stream, e->d_gates + b, e->d_args, e->d_words,
-                                           e->d_ones, e->geo, c));
+        const uint32_t it = ++e->iteration;
+        const uint32_t par = e->parity(it);
+        if (e->primed != it) {  // sample this pass on its own
+            mcs::PassLaunch pl = base_launch(e);
+            pl.samp_iteration = it;
+            pl.samp_row0 = par * n_ev;
+            pl.samp_begin = 0;
+            pl.samp_end = samp_total;
+            MCS_HIP(launch(e, pl, hook));
         }
+        const bool prefetch = e->n_buf == 2 && it != 0xFFFFFFFFu;
+        for (uint32_t l = 0; l < e->n_launches(); ++l) {
+            mcs::PassLaunch pl = base_launch(e);
+            pl.gates = e->d_gates + e->launch_gate_begin[l];
+            pl.arg_rows = e->d_args + uint64_t(par) * e->arg_span;
+            pl.n_gates = e->launch_gate_begin[l + 1] - e->launch_gate_begin[l];
+            if (prefetch) {
+                pl.samp_iteration = it + 1;
+                pl.samp_row0 = (par ^ 1u) * n_ev;
+                pl.samp_begin = e->launch_samp_begin[l];
+                pl.samp_end = e->launch_samp_begin[l + 1];
+            }
+            MCS_HIP(launch(e, pl, hook));
+        }
+        e->primed = prefetch ? it + 1 : 0;
     }
     return MCS_OK;
 }
@@ -236,19 +366,37 @@ mcs_status collect(mcs_engine *e, mcs_tally *out, int32_t n_out) {
         return fail(MCS_ERR_INVALID_ARGUMENT,
                     "collect needs exactly " + std::to_string(e->tally_nodes.size()) + " records");
     MCS_HIP(hipSetDevice(e->device));
-    std::vector<unsigned long long> ones(e->plan.n_nodes());
+    std::vector<unsigned long long> ones(e->n_rows);
     MCS_HIP(hipMemcpyAsync(ones.data(), e->d_ones, ones.size() * 8, hipMemcpyDeviceToHost, e->stream));
     MCS_HIP(hipStreamSynchronize(e->stream));
     for (size_t t = 0; t < e->tally_nodes.size(); ++t) {
         mcs_tally &r = out[t];
         r.index = e->tally_index[t];
-        r.num_one_bits = ones[e->tally_nodes[t]];
+        r.num_one_bits = ones[e->tally_slot(e->tally_nodes[t])];
         // total = iteration * N_bp * 64 (src/mc/kernel/tally.h:345-352).
         r.total_bits = uint64_t(e->iteration) * e->geo.n_bp * 64ull;
         fill_estimates(r);
     }
     return MCS_OK;
 }
+
+struct EventTimer : LaunchHook {
+    hipStream_t stream;
+    std::vector<hipEvent_t> ev;
+    explicit EventTimer(hipStream_t s) : stream(s) {}
+    hipError_t record() {
+        hipEvent_t x = nullptr;
+        hipError_t st = hipEventCreate(&x);
+        if (st != hipSuccess) return st;
+        ev.push_back(x);
+        return hipEventRecord(x, stream);
+    }
+    hipError_t before() override { return record(); }
+    hipError_t after() override { return record(); }
+    ~EventTimer() override {
+        for (hipEvent_t x : ev) (void)hipEventDestroy(x);
+    }
+};
 
 }  // namespace
 
@@ -320,6 +468,8 @@ mcs_status mcs_engine_get_info(mcs_engine *engine, mcs_engine_info *out) {
     out->num_tallies = int32_t(engine->tally_nodes.size());
     out->num_args = engine->n_args;
     out->device_bytes = engine->device_bytes;
+    out->event_buffers = int32_t(engine->n_buf);
+    out->launches_per_pass = engine->n_launches() + (engine->n_buf == 2 ? 0u : 1u);
     return MCS_OK;
 }
 
@@ -338,75 +488,43 @@ mcs_status mcs_engine_read_words(mcs_engine *engine, int32_t pdag_index, uint64_
     if (it == engine->node_of_index.end())
         return fail(MCS_ERR_GRAPH, "unknown node " + std::to_string(pdag_index));
     if (n_out < engine->geo.n_bp) return fail(MCS_ERR_INVALID_ARGUMENT, "output too small");
+    if (engine->iteration == 0) return fail(MCS_ERR_STATE, "no pass has run yet");
+    const int32_t node = it->second;
+    const uint64_t row = node < engine->plan.n_events()
+                             ? uint64_t(node) + uint64_t(engine->parity(engine->iteration)) *
+                                                    engine->plan.n_events()
+                             : engine->gate_row(node);
     MCS_HIP(hipSetDevice(engine->device));
-    MCS_HIP(hipMemcpyAsync(out, engine->d_words + uint64_t(it->second) * engine->geo.stride,
-                           engine->geo.n_bp * 8, hipMemcpyDeviceToHost, engine->stream));
+    MCS_HIP(hipMemcpyAsync(out, engine->d_words + row * engine->geo.stride, engine->geo.n_bp * 8,
+                           hipMemcpyDeviceToHost, engine->stream));
     MCS_HIP(hipStreamSynchronize(engine->stream));
     return MCS_OK;
 }
 
-mcs_status mcs_engine_time_passes(mcs_engine *engine, uint32_t n_passes, double *sampler_ms,
-                                  double *gates_ms, double *pass_ms) {
-    if (!engine || !sampler_ms || !gates_ms || !pass_ms)
+mcs_status mcs_engine_time_passes(mcs_engine *engine, uint32_t n_passes, double *kernel_ms,
+                                  uint32_t *launches, double *pass_ms) {
+    if (!engine || !kernel_ms || !launches || !pass_ms)
         return fail(MCS_ERR_INVALID_ARGUMENT, "null argument");
     if (n_passes == 0) return fail(MCS_ERR_INVALID_ARGUMENT, "n_passes must be positive");
     MCS_HIP(hipSetDevice(engine->device));
-    const mcs::Plan &plan = engine->plan;
-    // Three events per pass: before the sampler, between sampler and gates,
-    // after the gates.
-    std::vector<hipEvent_t> ev(size_t(n_passes) * 3, nullptr);
-    auto release = [&]() {
-        for (hipEvent_t x : ev)
-            if (x) (void)hipEventDestroy(x);
-    };
-    for (hipEvent_t &x : ev) {
-        const hipError_t e = hipEventCreate(&x);
-        if (e != hipSuccess) {
-            release();
-            return fail(MCS_ERR_DEVICE, std::string("hipEventCreate failed: ") + hipGetErrorString(e));
-        }
+    EventTimer timer(engine->stream);
+    mcs_status st = enqueue_passes(engine, n_passes, &timer);
+    if (st != MCS_OK) return st;
+    MCS_HIP(hipStreamSynchronize(engine->stream));
+    double sum = 0;
+    float ms = 0;
+    for (size_t i = 0; i + 1 < timer.ev.size(); i += 2) {
+        MCS_HIP(hipEventElapsedTime(&ms, timer.ev[i], timer.ev[i + 1]));
+        sum += ms;
     }
-    mcs_status st = MCS_OK;
-    for (uint32_t p = 0; p < n_passes && st == MCS_OK; ++p) {
-        if (engine->iteration == 0xFFFFFFFFu) {
-            st = fail(MCS_ERR_STATE, "iteration counter exhausted");
-            break;
-        }
-        const uint32_t iteration = ++engine->iteration;
-        hipError_t e = hipEventRecord(ev[3 * p], engine->stream);
-        if (e == hipSuccess)
-            e = mcs::launch_sample_events(engine->stream, engine->d_events, engine->d_words,
-                                          engine->d_ones, engine->geo, uint32_t(plan.n_events()),
-                                          iteration, engine->key0, engine->key1);
-        if (e == hipSuccess) e = hipEventRecord(ev[3 * p + 1], engine->stream);
-        for (int32_t l = 0; l < plan.n_gate_layers() && e == hipSuccess; ++l) {
-            const uint32_t b = plan.layer_begin[l], c = plan.layer_begin[l + 1] - b;
-            e = mcs::launch_eval_gates(engine->stream, engine->d_gates + b, engine->d_args,
-                                       engine->d_words, engine->d_ones, engine->geo, c);
-        }
-        if (e == hipSuccess) e = hipEventRecord(ev[3 * p + 2], engine->stream);
-        if (e != hipSuccess) st = fail(MCS_ERR_DEVICE, std::string("pass launch failed: ") + hipGetErrorString(e));
-    }
-    if (st == MCS_OK) {
-        const hipError_t e = hipStreamSynchronize(engine->stream);
-        if (e != hipSuccess) st = fail(MCS_ERR_DEVICE, std::string("hipStreamSynchronize failed: ") + hipGetErrorString(e));
-    }
-    if (st == MCS_OK) {
-        double s = 0, g = 0;
-        float ms = 0;
-        for (uint32_t p = 0; p < n_passes; ++p) {
-            (void)hipEventElapsedTime(&ms, ev[3 * p], ev[3 * p + 1]);
-            s += ms;
-            (void)hipEventElapsedTime(&ms, ev[3 * p + 1], ev[3 * p + 2]);
-            g += ms;
-        }
-        (void)hipEventElapsedTime(&ms, ev[0], ev[3 * (n_passes - 1) + 2]);
-        *sampler_ms = s;
-        *gates_ms = g;
+    *kernel_ms = sum;
+    *launches = uint32_t(timer.ev.size() / 2);
+    *pass_ms = 0;
+    if (timer.ev.size() >= 2) {
+        MCS_HIP(hipEventElapsedTime(&ms, timer.ev.front(), timer.ev.back()));
         *pass_ms = ms;
     }
-    release();
-    return st;
+    return MCS_OK;
 }
 
 mcs_status mcs_engine_restore(mcs_engine *engine, uint32_t iteration, const uint64_t *num_one_bits,
@@ -415,11 +533,12 @@ mcs_status mcs_engine_restore(mcs_engine *engine, uint32_t iteration, const uint
     if (n != int32_t(engine->tally_nodes.size()))
         return fail(MCS_ERR_INVALID_ARGUMENT, "restore needs one count per tally");
     MCS_HIP(hipSetDevice(engine->device));
-    std::vector<unsigned long long> ones(engine->plan.n_nodes(), 0);
-    for (int32_t t = 0; t < n; ++t) ones[engine->tally_nodes[t]] = num_one_bits[t];
+    std::vector<unsigned long long> ones(engine->n_rows, 0);
+    for (int32_t t = 0; t < n; ++t) ones[engine->tally_slot(engine->tally_nodes[t])] = num_one_bits[t];
     MCS_HIP(hipStreamSynchronize(engine->stream));
     MCS_HIP(hipMemcpy(engine->d_ones, ones.data(), ones.size() * 8, hipMemcpyHostToDevice));
     engine->iteration = iteration;
+    engine->primed = 0;
     return MCS_OK;
 }
 

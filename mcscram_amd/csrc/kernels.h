@@ -1,9 +1,13 @@
 // Device-side data layout shared by the engine's host code and its kernels.
 //
 // HBM layout of one engine: a single row-major matrix of uint64 words,
-// one row per node, `stride` words per row. Rows [0, n_events) hold the
-// basic events in event_idx order; gate rows follow layer by layer. Word s
-// of a row covers trials 64*s .. 64*s+63 of the current pass.
+// `stride` words per row. Word s of a row covers trials 64*s .. 64*s+63 of
+// one pass. Rows:
+//   [0, n_ev)              basic events of buffer 0, in event_idx order
+//   [n_ev, 2 n_ev)         basic events of buffer 1 (when double-buffered)
+//   [n_buf n_ev, ...)      gates, layer by layer
+// Pass i evaluates its gates from buffer (i & 1) while the same launches
+// sample pass i + 1 into the other buffer.
 #pragma once
 
 #include <cstdint>
@@ -20,25 +24,43 @@ constexpr uint32_t kPhiloxW1 = 0xBB67AE85u;
 // drawing (the drawn value is always < 2^32).
 enum : uint32_t { kEventDraw = 0, kEventNever = 1, kEventAlways = 2 };
 
-struct EventDesc {       // 16 bytes, one per basic event, event_idx order
-    uint32_t ctr0;       // PDAG index + 1       (Philox counter word 0)
-    uint32_t ctr1;       // event_idx + 1         (Philox counter word 1)
-    uint32_t threshold;  // p * 2^32, truncated, when mode == kEventDraw
-    uint32_t mode_tally; // mode | tallied << 2; tallies accumulate at ones[row]
+struct EventDesc {     // 16 bytes, one per basic event, event_idx order
+    uint32_t ctr0;     // PDAG index + 1       (Philox counter word 0)
+    uint32_t ctr1;     // event_idx + 1         (Philox counter word 1)
+    uint32_t threshold;// p * 2^32, truncated, when mode == kEventDraw
+    uint32_t mode;
 };
 
-struct GateDesc {        // 16 bytes, one per gate, evaluation order
-    uint32_t out_row;
-    uint32_t arg_begin;  // into the argument-row array
-    uint32_t counts;     // n_in | n_pos << 16  (positive inputs come first)
-    uint32_t type_k_tally; // connective | k8 << 8 | tallied << 16
+// Gate flags (GateDesc::type_k_flags bits 16..).
+constexpr uint32_t kGateTallied = 1u << 16;  // popcount into ones[out_row]
+constexpr uint32_t kGateNoStore = 1u << 17;  // tally only (basic-event tallies)
+
+struct GateDesc {          // 16 bytes, one per gate, evaluation order
+    uint32_t out_row;      // row written; also the tally slot
+    uint32_t arg_begin;    // into the argument-row array
+    uint32_t counts;       // n_in | n_pos << 16  (positive inputs come first)
+    uint32_t type_k_flags; // connective | k8 << 8 | flags
 };
 
 struct LaunchGeometry {
     uint64_t stride;        // words per row (multiple of chunk_words)
     uint64_t n_bp;          // live words per row; words >= n_bp are padding
-    uint32_t chunk_words;   // words per workgroup task (multiple of 512)
+    uint32_t chunk_words;   // words per workgroup task (512, 1024 or 2048)
     uint32_t n_chunks;      // stride / chunk_words
+};
+
+// One launch of the pass kernel: the gates of one layer over every chunk,
+// plus a contiguous range of sampler tasks (event-major (event, chunk)
+// pairs) for the pass whose basic events go to rows samp_row0 + e.
+struct PassLaunch {
+    const GateDesc *gates;
+    const uint32_t *arg_rows;
+    uint32_t n_gates;
+    uint32_t samp_iteration;
+    const EventDesc *events;
+    uint64_t samp_begin, samp_end;
+    uint32_t samp_row0;
+    uint32_t key0, key1;
 };
 
 }  // namespace mcs
@@ -47,14 +69,9 @@ struct LaunchGeometry {
 
 namespace mcs {
 
-// Launch wrappers (defined in kernels.hip). Grids are n_events x n_chunks and
-// n_gates x n_chunks workgroups of 256 lanes; the caller keeps both products
-// below 2^32 / 256.
-hipError_t launch_sample_events(hipStream_t stream, const EventDesc *events, uint64_t *words,
-                                unsigned long long *ones, const LaunchGeometry &geo,
-                                uint32_t n_events, uint32_t iteration, uint32_t k0, uint32_t k1);
-hipError_t launch_eval_gates(hipStream_t stream, const GateDesc *gates, const uint32_t *arg_rows,
-                             uint64_t *words, unsigned long long *ones, const LaunchGeometry &geo,
-                             uint32_t n_gates);
+// Blocks of 256 lanes; the caller checks grid_blocks(...) * 256 < 2^32.
+uint64_t pass_grid_blocks(const PassLaunch &l, const LaunchGeometry &geo);
+hipError_t launch_pass(hipStream_t stream, const PassLaunch &l, uint64_t *words,
+                       unsigned long long *ones, const LaunchGeometry &geo);
 
 }  // namespace mcs

@@ -1,12 +1,14 @@
-// HIP kernels of the Monte Carlo fault-tree engine, written for CDNA4 (gfx950).
+// HIP kernel of the Monte Carlo fault-tree engine, written for CDNA4 (gfx950).
 //
-//   sample_events  Philox4x32-10 Bernoulli sampler, one uint64 word = 64
-//                  trials per (event, word); lanes own consecutive words of
-//                  one event so every per-event value sits in SGPRs and the
-//                  stores are coalesced 16-byte writes.
-//   eval_gates     one launch per layer: every gate of the layer over a chunk
-//                  of words, 16-byte loads of each input row, bit-sliced
+// One kernel, pass_layer, does both kinds of work of a pass; each launch
+// mixes them so the VALU-bound sampler and the HBM-bound gate evaluation
+// share every CU:
+//   gate tasks     one gate of the current layer over a chunk of words:
+//                  16-byte loads of each input row, bitwise fold, bit-sliced
 //                  counter for ATLEAST, popcount tally fused at production.
+//   sampler tasks  Philox4x32-10 Bernoulli words of one basic event over a
+//                  chunk of words; lanes own consecutive words so every
+//                  per-event value sits in SGPRs and stores are coalesced.
 //
 // Reference semantics restated here:
 //   counter (pdag_idx+1, event_idx+1, sample_idx+1, ((iteration+1)<<6)+g),
@@ -21,6 +23,9 @@
 
 namespace mcs {
 namespace {
+
+constexpr uint32_t kLanes = 256;
+constexpr uint32_t kXcds = 8;  // workgroups are dealt to the 8 XCDs round-robin
 
 __device__ __forceinline__ void mul_wide(uint32_t a, uint32_t m, uint32_t &hi, uint32_t &lo) {
     const uint64_t p = uint64_t(a) * uint64_t(m);
@@ -52,8 +57,11 @@ __device__ __forceinline__ void philox_round(uint32_t &x0, uint32_t &x1, uint32_
 // same for all 16 generations and are computed once per word, and the
 // generation-dependent half of round 2 depends only on per-event values
 // (c0, c3base, g), which are wave-uniform here and go to the scalar unit.
-// That leaves 15 vector multiplies per generation plus 3 per word instead
-// of 20 per generation.
+//
+// Packing: the four draws of a generation hit only if their minimum is below
+// the threshold; when no lane of the wave hits (most generations at the
+// small probabilities of fault trees) the compares and shifts are skipped by
+// one wave-uniform branch.
 __device__ __forceinline__ uint64_t bernoulli_word(uint32_t c0, uint32_t c1, uint32_t c2,
                                                    uint32_t c3base, uint32_t k0, uint32_t k1,
                                                    uint32_t thr) {
@@ -93,12 +101,15 @@ __device__ __forceinline__ uint64_t bernoulli_word(uint32_t c0, uint32_t c1, uin
 #pragma unroll
         for (uint32_t r = 3; r < 10; ++r)
             philox_round(x0, x1, x2, x3, k0 + r * kPhiloxW0, k1 + r * kPhiloxW1);
-        const uint32_t nib = uint32_t(x0 < thr) | uint32_t(x1 < thr) << 1 |
-                             uint32_t(x2 < thr) << 2 | uint32_t(x3 < thr) << 3;
-        if (g < 8)
-            lo_bits |= nib << (4u * g);
-        else
-            hi_bits |= nib << (4u * (g - 8u));
+        const uint32_t m = min(min(x0, x1), min(x2, x3));
+        if (__builtin_amdgcn_ballot_w64(m < thr) != 0) {
+            const uint32_t nib = uint32_t(x0 < thr) | uint32_t(x1 < thr) << 1 |
+                                 uint32_t(x2 < thr) << 2 | uint32_t(x3 < thr) << 3;
+            if (g < 8)
+                lo_bits |= nib << (4u * g);
+            else
+                hi_bits |= nib << (4u * (g - 8u));
+        }
     }
     return uint64_t(hi_bits) << 32 | lo_bits;
 }
@@ -114,202 +125,250 @@ __device__ __forceinline__ uint32_t live_popcount(uint64_t w0, uint64_t w1, uint
     return (s < n_bp ? __popcll(w0) : 0u) + (s + 1 < n_bp ? __popcll(w1) : 0u);
 }
 
-}  // namespace
-
-// grid.x = n_events * n_chunks (event-major), block = 256 lanes, 2 words/lane.
-__global__ __launch_bounds__(256) void sample_events(const EventDesc *__restrict__ events,
-                                                     uint64_t *__restrict__ words,
-                                                     unsigned long long *__restrict__ ones,
-                                                     LaunchGeometry geo, uint32_t n_events,
-                                                     uint32_t iteration, uint32_t k0,
-                                                     uint32_t k1) {
-    const uint32_t e = blockIdx.x / geo.n_chunks;
-    const uint32_t chunk = blockIdx.x - e * geo.n_chunks;
-    if (e >= n_events) return;
-    const EventDesc d = events[e];
-    const uint32_t mode = d.mode_tally & 3u;
-    const bool tallied = (d.mode_tally >> 2) & 1u;
-    const uint32_t c3base = (iteration + 1u) << 6;
-    uint64_t *row = words + uint64_t(e) * geo.stride;
-    const uint64_t base = uint64_t(chunk) * geo.chunk_words;
-    uint32_t pc = 0;
-    for (uint32_t off = threadIdx.x * 2u; off < geo.chunk_words; off += 512u) {
-        const uint64_t s = base + off;
-        uint64_t w0, w1;
-        if (mode == kEventDraw) {
-            w0 = bernoulli_word(d.ctr0, d.ctr1, uint32_t(s) + 1u, c3base, k0, k1, d.threshold);
-            w1 = bernoulli_word(d.ctr0, d.ctr1, uint32_t(s) + 2u, c3base, k0, k1, d.threshold);
-        } else {
-            w0 = w1 = (mode == kEventAlways) ? ~0ull : 0ull;
-        }
-        *reinterpret_cast<ulonglong2 *>(row + s) = make_ulonglong2(w0, w1);
-        if (tallied) pc += live_popcount(w0, w1, s, geo.n_bp);
-    }
-    if (tallied) {
-        pc = wave_sum(pc);
-        if ((threadIdx.x & 63u) == 0 && pc) atomicAdd(ones + e, (unsigned long long)pc);
-    }
+__device__ __forceinline__ void tally_wave(unsigned long long *ones, uint32_t slot, uint32_t pc) {
+    pc = wave_sum(pc);
+    if ((threadIdx.x & 63u) == 0 && pc) atomicAdd(ones + slot, (unsigned long long)pc);
 }
 
-namespace {
+// Sampler task: words [base, base + chunk_words) of one event.
+__device__ __forceinline__ void sampler_task(const PassLaunch &L, uint64_t *__restrict__ words,
+                                             const LaunchGeometry &geo, uint64_t task) {
+    const uint32_t e = uint32_t(task / geo.n_chunks);
+    const uint32_t chunk = uint32_t(task - uint64_t(e) * geo.n_chunks);
+    const EventDesc d = L.events[e];
+    const uint32_t c3base = (L.samp_iteration + 1u) << 6;
+    uint64_t *row = words + uint64_t(L.samp_row0 + e) * geo.stride;
+    const uint64_t base = uint64_t(chunk) * geo.chunk_words;
+    for (uint32_t off = threadIdx.x * 2u; off < geo.chunk_words; off += 2u * kLanes) {
+        const uint64_t s = base + off;
+        uint64_t w0, w1;
+        if (d.mode == kEventDraw) {
+            w0 = bernoulli_word(d.ctr0, d.ctr1, uint32_t(s) + 1u, c3base, L.key0, L.key1, d.threshold);
+            w1 = bernoulli_word(d.ctr0, d.ctr1, uint32_t(s) + 2u, c3base, L.key0, L.key1, d.threshold);
+        } else {
+            w0 = w1 = (d.mode == kEventAlways) ? ~0ull : 0ull;
+        }
+        *reinterpret_cast<ulonglong2 *>(row + s) = make_ulonglong2(w0, w1);
+    }
+}
 
 struct Pair {
     uint64_t x, y;
 };
 
-__device__ __forceinline__ Pair load_pair(const uint64_t *__restrict__ words, uint32_t row,
-                                          uint64_t stride, uint64_t s) {
-    const ulonglong2 v = *reinterpret_cast<const ulonglong2 *>(words + uint64_t(row) * stride + s);
+__device__ __forceinline__ Pair load_pair(const uint64_t *__restrict__ p) {
+    const ulonglong2 v = *reinterpret_cast<const ulonglong2 *>(p);
     return {v.x, v.y};
 }
 
-// Fold the inputs [positives..., negated...] with op; loads are issued four
-// at a time so several rows are in flight per wave.
-template <typename Op>
-__device__ __forceinline__ void fold_inputs(const uint32_t *__restrict__ a, uint32_t n_in,
-                                            uint32_t n_pos, const uint64_t *__restrict__ words,
-                                            uint64_t stride, uint64_t s, Op &&op) {
+// AND / OR / XOR families over K pairs per lane (lane words off + 512 k):
+// inputs two at a time so 2K 16-byte loads are in flight per lane.
+template <int K, int Op>  // Op: 0 AND, 1 OR, 2 XOR
+__device__ __forceinline__ void fold_simple(Pair (&r)[K], const uint32_t *__restrict__ a,
+                                            uint32_t n_in, uint32_t n_pos,
+                                            const uint64_t *__restrict__ words, uint64_t stride,
+                                            uint64_t s0) {
+    auto apply = [&](Pair &acc, Pair v) {
+        if (Op == 0) { acc.x &= v.x; acc.y &= v.y; }
+        if (Op == 1) { acc.x |= v.x; acc.y |= v.y; }
+        if (Op == 2) { acc.x ^= v.x; acc.y ^= v.y; }
+    };
     uint32_t i = 0;
-    for (; i + 4 <= n_in; i += 4) {
-        Pair v[4];
+    for (; i + 2 <= n_in; i += 2) {
+        const uint64_t *p0 = words + uint64_t(a[i]) * stride + s0;
+        const uint64_t *p1 = words + uint64_t(a[i + 1]) * stride + s0;
+        Pair v0[K], v1[K];
 #pragma unroll
-        for (int j = 0; j < 4; ++j) v[j] = load_pair(words, a[i + j], stride, s);
+        for (int k = 0; k < K; ++k) {
+            v0[k] = load_pair(p0 + 2u * kLanes * k);
+            v1[k] = load_pair(p1 + 2u * kLanes * k);
+        }
+        const bool n0 = i >= n_pos, n1 = i + 1 >= n_pos;
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            if (i + j >= n_pos) {
-                v[j].x = ~v[j].x;
-                v[j].y = ~v[j].y;
+        for (int k = 0; k < K; ++k) {
+            if (n0) { v0[k].x = ~v0[k].x; v0[k].y = ~v0[k].y; }
+            if (n1) { v1[k].x = ~v1[k].x; v1[k].y = ~v1[k].y; }
+            apply(r[k], v0[k]);
+            apply(r[k], v1[k]);
+        }
+    }
+    if (i < n_in) {
+        const uint64_t *p0 = words + uint64_t(a[i]) * stride + s0;
+        const bool n0 = i >= n_pos;
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            Pair v = load_pair(p0 + 2u * kLanes * k);
+            if (n0) { v.x = ~v.x; v.y = ~v.y; }
+            apply(r[k], v);
+        }
+    }
+}
+
+// ATLEAST on one pair per lane: bit-sliced counter (carry-save planes), then
+// the per-bit comparison count >= k8 on the planes. Eight planes hold the
+// reference's 8-bit count, so counts wrap at 256 exactly as it does.
+__device__ __forceinline__ Pair atleast_pair(const uint32_t *__restrict__ a, uint32_t n_in,
+                                             uint32_t n_pos, uint32_t k8,
+                                             const uint64_t *__restrict__ words, uint64_t stride,
+                                             uint64_t s) {
+    const uint32_t planes = n_in >= 255u ? 8u : 32u - __clz(n_in);
+    Pair c[8];
+#pragma unroll
+    for (int b = 0; b < 8; ++b) c[b] = {0, 0};
+    for (uint32_t i = 0; i < n_in; ++i) {
+        Pair carry = load_pair(words + uint64_t(a[i]) * stride + s);
+        if (i >= n_pos) { carry.x = ~carry.x; carry.y = ~carry.y; }
+#pragma unroll
+        for (uint32_t b = 0; b < 8; ++b) {
+            if (b < planes) {
+                const Pair t = {c[b].x & carry.x, c[b].y & carry.y};
+                c[b].x ^= carry.x;
+                c[b].y ^= carry.y;
+                carry = t;
             }
-            op(v[j]);
         }
     }
-    for (; i < n_in; ++i) {
-        Pair v = load_pair(words, a[i], stride, s);
-        if (i >= n_pos) {
-            v.x = ~v.x;
-            v.y = ~v.y;
+    Pair gt = {0, 0}, eq = {~0ull, ~0ull};
+#pragma unroll
+    for (int b = 7; b >= 0; --b) {
+        if ((k8 >> b) & 1u) {
+            eq.x &= c[b].x;
+            eq.y &= c[b].y;
+        } else {
+            gt.x |= eq.x & c[b].x;
+            gt.y |= eq.y & c[b].y;
+            eq.x &= ~c[b].x;
+            eq.y &= ~c[b].y;
         }
-        op(v);
     }
+    return {gt.x | eq.x, gt.y | eq.y};
+}
+
+template <int K>
+__device__ __forceinline__ void gate_task_k(const GateDesc &d, const uint32_t *__restrict__ a,
+                                            uint64_t *__restrict__ words,
+                                            unsigned long long *__restrict__ ones,
+                                            const LaunchGeometry &geo, uint64_t base) {
+    const uint32_t n_in = d.counts & 0xFFFFu;
+    const uint32_t n_pos = d.counts >> 16;
+    const uint32_t type = d.type_k_flags & 0xFFu;
+    const uint32_t k8 = (d.type_k_flags >> 8) & 0xFFu;
+    const uint64_t s0 = base + threadIdx.x * 2u;
+    Pair r[K];
+    if (type == 2u) {  // ATLEAST
+#pragma unroll
+        for (int k = 0; k < K; ++k)
+            r[k] = atleast_pair(a, n_in, n_pos, k8, words, geo.stride, s0 + 2u * kLanes * k);
+    } else if (type == 4u || type == 7u) {  // NOT, NULL: the last input
+#pragma unroll
+        for (int k = 0; k < K; ++k) r[k] = {0, 0};
+        if (n_in) {
+            const uint64_t *p = words + uint64_t(a[n_in - 1]) * geo.stride + s0;
+#pragma unroll
+            for (int k = 0; k < K; ++k) {
+                r[k] = load_pair(p + 2u * kLanes * k);
+                if (n_in - 1 >= n_pos) { r[k].x = ~r[k].x; r[k].y = ~r[k].y; }
+            }
+        }
+    } else {
+        const bool and_like = (type == 0u || type == 5u);
+#pragma unroll
+        for (int k = 0; k < K; ++k) r[k] = and_like ? Pair{~0ull, ~0ull} : Pair{0, 0};
+        if (and_like)
+            fold_simple<K, 0>(r, a, n_in, n_pos, words, geo.stride, s0);
+        else if (type == 3u)
+            fold_simple<K, 2>(r, a, n_in, n_pos, words, geo.stride, s0);
+        else
+            fold_simple<K, 1>(r, a, n_in, n_pos, words, geo.stride, s0);
+    }
+    if (type == 4u || type == 5u || type == 6u) {  // NOT, NAND, NOR
+#pragma unroll
+        for (int k = 0; k < K; ++k) { r[k].x = ~r[k].x; r[k].y = ~r[k].y; }
+    }
+    if (!(d.type_k_flags & kGateNoStore)) {
+        uint64_t *out = words + uint64_t(d.out_row) * geo.stride + s0;
+#pragma unroll
+        for (int k = 0; k < K; ++k)
+            *reinterpret_cast<ulonglong2 *>(out + 2u * kLanes * k) = make_ulonglong2(r[k].x, r[k].y);
+    }
+    if (d.type_k_flags & kGateTallied) {
+        uint32_t pc = 0;
+#pragma unroll
+        for (int k = 0; k < K; ++k) pc += live_popcount(r[k].x, r[k].y, s0 + 2u * kLanes * k, geo.n_bp);
+        tally_wave(ones, d.out_row, pc);
+    }
+}
+
+// Gate task: one gate over words [base, base + chunk_words).
+__device__ __forceinline__ void gate_task(const PassLaunch &L, uint64_t *__restrict__ words,
+                                          unsigned long long *__restrict__ ones,
+                                          const LaunchGeometry &geo, uint64_t task) {
+    const uint32_t chunk = uint32_t(task / L.n_gates);
+    const uint32_t gi = uint32_t(task - uint64_t(chunk) * L.n_gates);
+    const GateDesc d = L.gates[gi];
+    const uint32_t *a = L.arg_rows + d.arg_begin;
+    const uint64_t base = uint64_t(chunk) * geo.chunk_words;
+    switch (geo.chunk_words) {
+    case 2048: gate_task_k<4>(d, a, words, ones, geo, base); break;
+    case 1024: gate_task_k<2>(d, a, words, ones, geo, base); break;
+    default: gate_task_k<1>(d, a, words, ones, geo, base); break;
+    }
+}
+
+// Per-XCD split of a launch: workgroup b runs on XCD b % 8 as its (b / 8)-th
+// workgroup there. XCD x owns the x-th eighth of the gate tasks (chunk-major,
+// so the gates it evaluates together read the same chunk of their input rows
+// through its own L2) and the x-th eighth of the sampler tasks, and deals
+// them out interleaved in proportion.
+struct Split {
+    uint64_t g_lo, s_lo, ngx, tx;
+};
+
+__host__ __device__ __forceinline__ Split split_for(uint64_t ng, uint64_t ns, uint32_t x) {
+    Split s;
+    s.g_lo = ng * x / kXcds;
+    s.s_lo = ns * x / kXcds;
+    s.ngx = ng * (x + 1) / kXcds - s.g_lo;
+    s.tx = s.ngx + (ns * (x + 1) / kXcds - s.s_lo);
+    return s;
 }
 
 }  // namespace
 
-// grid.x = n_chunks * n_gates (chunk-major: blocks running together share a
-// chunk, so input rows read by several gates of the layer are reused from
-// cache), block = 256 lanes, 2 words/lane.
-__global__ __launch_bounds__(256) void eval_gates(const GateDesc *__restrict__ gates,
-                                                  const uint32_t *__restrict__ arg_rows,
-                                                  uint64_t *__restrict__ words,
+__global__ __launch_bounds__(256) void pass_layer(PassLaunch L, uint64_t *__restrict__ words,
                                                   unsigned long long *__restrict__ ones,
-                                                  LaunchGeometry geo, uint32_t n_gates) {
-    const uint32_t chunk = blockIdx.x / n_gates;
-    const uint32_t gi = blockIdx.x - chunk * n_gates;
-    if (chunk >= geo.n_chunks) return;
-    const GateDesc d = gates[gi];
-    const uint32_t n_in = d.counts & 0xFFFFu;
-    const uint32_t n_pos = d.counts >> 16;
-    const uint32_t type = d.type_k_tally & 0xFFu;
-    const uint32_t k8 = (d.type_k_tally >> 8) & 0xFFu;
-    const bool tallied = (d.type_k_tally >> 16) & 1u;
-    const uint32_t *a = arg_rows + d.arg_begin;
-    uint64_t *out = words + uint64_t(d.out_row) * geo.stride;
-    const uint64_t base = uint64_t(chunk) * geo.chunk_words;
-    uint32_t pc = 0;
-    for (uint32_t off = threadIdx.x * 2u; off < geo.chunk_words; off += 512u) {
-        const uint64_t s = base + off;
-        Pair r;
-        if (type == 2u) {  // ATLEAST: bit-sliced 8-bit counter, then count >= k8
-            const uint32_t planes = n_in >= 255u ? 8u : 32u - __clz(n_in);
-            Pair c[8];
-#pragma unroll
-            for (int b = 0; b < 8; ++b) c[b] = {0, 0};
-            fold_inputs(a, n_in, n_pos, words, geo.stride, s, [&](Pair v) {
-                Pair carry = v;
-#pragma unroll
-                for (uint32_t b = 0; b < 8; ++b) {
-                    if (b < planes) {
-                        const Pair t = {c[b].x & carry.x, c[b].y & carry.y};
-                        c[b].x ^= carry.x;
-                        c[b].y ^= carry.y;
-                        carry = t;
-                    }
-                }
-            });
-            Pair gt = {0, 0}, eq = {~0ull, ~0ull};
-#pragma unroll
-            for (int b = 7; b >= 0; --b) {
-                if ((k8 >> b) & 1u) {
-                    eq.x &= c[b].x;
-                    eq.y &= c[b].y;
-                } else {
-                    gt.x |= eq.x & c[b].x;
-                    gt.y |= eq.y & c[b].y;
-                    eq.x &= ~c[b].x;
-                    eq.y &= ~c[b].y;
-                }
-            }
-            r = {gt.x | eq.x, gt.y | eq.y};
-        } else {
-            const bool and_like = (type == 0u || type == 5u);  // AND, NAND
-            r = and_like ? Pair{~0ull, ~0ull} : Pair{0, 0};
-            switch (type) {
-            case 0u: case 5u:  // AND, NAND
-                fold_inputs(a, n_in, n_pos, words, geo.stride, s, [&](Pair v) { r.x &= v.x; r.y &= v.y; });
-                break;
-            case 1u: case 6u:  // OR, NOR
-                fold_inputs(a, n_in, n_pos, words, geo.stride, s, [&](Pair v) { r.x |= v.x; r.y |= v.y; });
-                break;
-            case 3u:  // XOR
-                fold_inputs(a, n_in, n_pos, words, geo.stride, s, [&](Pair v) { r.x ^= v.x; r.y ^= v.y; });
-                break;
-            default:  // NOT, NULL: the last input in [positives..., negated...] order
-                if (n_in) {
-                    r = load_pair(words, a[n_in - 1], geo.stride, s);
-                    if (n_in - 1 >= n_pos) {
-                        r.x = ~r.x;
-                        r.y = ~r.y;
-                    }
-                }
-                break;
-            }
-            if (type == 4u || type == 5u || type == 6u) {  // NOT, NAND, NOR
-                r.x = ~r.x;
-                r.y = ~r.y;
-            }
-        }
-        *reinterpret_cast<ulonglong2 *>(out + s) = make_ulonglong2(r.x, r.y);
-        if (tallied) pc += live_popcount(r.x, r.y, s, geo.n_bp);
-    }
-    if (tallied) {
-        pc = wave_sum(pc);
-        if ((threadIdx.x & 63u) == 0 && pc) atomicAdd(ones + d.out_row, (unsigned long long)pc);
-    }
+                                                  LaunchGeometry geo) {
+    const uint64_t ng = uint64_t(L.n_gates) * geo.n_chunks;
+    const uint64_t ns = L.samp_end - L.samp_begin;
+    const uint32_t x = blockIdx.x % kXcds;
+    const uint64_t j = blockIdx.x / kXcds;
+    const Split sp = split_for(ng, ns, x);
+    if (j >= sp.tx) return;
+    const uint64_t before = j * sp.ngx / sp.tx;
+    const uint64_t upto = (j + 1) * sp.ngx / sp.tx;
+    if (upto > before)
+        gate_task(L, words, ones, geo, sp.g_lo + before);
+    else
+        sampler_task(L, words, geo, L.samp_begin + sp.s_lo + (j - before));
 }
 
-}  // namespace mcs
-
-// Host-side launch wrappers (the engine's other translation units are host
-// code and launch through these).
-namespace mcs {
-
-hipError_t launch_sample_events(hipStream_t stream, const EventDesc *events, uint64_t *words,
-                                unsigned long long *ones, const LaunchGeometry &geo,
-                                uint32_t n_events, uint32_t iteration, uint32_t k0, uint32_t k1) {
-    if (n_events == 0) return hipSuccess;
-    const uint64_t blocks = uint64_t(n_events) * geo.n_chunks;
-    hipLaunchKernelGGL(sample_events, dim3(uint32_t(blocks)), dim3(256), 0, stream, events, words,
-                       ones, geo, n_events, iteration, k0, k1);
-    return hipGetLastError();
+uint64_t pass_grid_blocks(const PassLaunch &l, const LaunchGeometry &geo) {
+    const uint64_t ng = uint64_t(l.n_gates) * geo.n_chunks;
+    const uint64_t ns = l.samp_end - l.samp_begin;
+    uint64_t m = 0;
+    for (uint32_t x = 0; x < kXcds; ++x) {
+        const uint64_t t = split_for(ng, ns, x).tx;
+        m = t > m ? t : m;
+    }
+    return m * kXcds;
 }
 
-hipError_t launch_eval_gates(hipStream_t stream, const GateDesc *gates, const uint32_t *arg_rows,
-                             uint64_t *words, unsigned long long *ones, const LaunchGeometry &geo,
-                             uint32_t n_gates) {
-    if (n_gates == 0) return hipSuccess;
-    const uint64_t blocks = uint64_t(n_gates) * geo.n_chunks;
-    hipLaunchKernelGGL(eval_gates, dim3(uint32_t(blocks)), dim3(256), 0, stream, gates, arg_rows,
-                       words, ones, geo, n_gates);
+hipError_t launch_pass(hipStream_t stream, const PassLaunch &l, uint64_t *words,
+                       unsigned long long *ones, const LaunchGeometry &geo) {
+    const uint64_t blocks = pass_grid_blocks(l, geo);
+    if (blocks == 0) return hipSuccess;
+    hipLaunchKernelGGL(pass_layer, dim3(uint32_t(blocks)), dim3(kLanes), 0, stream, l, words, ones,
+                       geo);
     return hipGetLastError();
 }
 

@@ -50,7 +50,8 @@ class _Info(C.Structure):
     _fields_ = [("num_bitpacks", C.c_uint64), ("row_stride", C.c_uint64),
                 ("iteration", C.c_uint32), ("num_events", C.c_int32), ("num_gates", C.c_int32),
                 ("num_layers", C.c_int32), ("num_tallies", C.c_int32), ("num_args", C.c_uint64),
-                ("device_bytes", C.c_uint64)]
+                ("device_bytes", C.c_uint64), ("event_buffers", C.c_int32),
+                ("launches_per_pass", C.c_uint32)]
 
 
 class _Settings(C.Structure):
@@ -224,11 +225,13 @@ class LayerManager:
         return out
 
     def time_passes(self, n: int) -> dict:
-        """Run n passes with HIP events on the engine's stream; device ms."""
-        s, g, t = C.c_double(), C.c_double(), C.c_double()
-        _check(load_library().mcs_engine_time_passes(self._h, n, C.byref(s), C.byref(g),
+        """Run n passes with HIP events around every launch on the engine's
+        stream; returns summed kernel ms, launch count and first-to-last ms."""
+        k, t = C.c_double(), C.c_double()
+        launches = C.c_uint32()
+        _check(load_library().mcs_engine_time_passes(self._h, n, C.byref(k), C.byref(launches),
                                                      C.byref(t)))
-        return {"sampler_ms": s.value, "gates_ms": g.value, "pass_ms": t.value}
+        return {"kernel_ms": k.value, "launches": launches.value, "pass_ms": t.value}
 
     def restore(self, iteration: int, num_one_bits) -> None:
         ones = np.ascontiguousarray(num_one_bits, dtype=np.uint64)

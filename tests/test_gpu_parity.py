@@ -161,3 +161,32 @@ def test_event_order_on_engine_matches_host_plan(gpu):
     pd = mc.small_tree(1)
     with mc.LayerManager(pd, 64) as eng:
         assert np.array_equal(eng.event_order(), mc.pdag_event_order(pd))
+
+
+def test_passes_in_separate_calls_equal_one_call(gpu):
+    # The launches of pass i sample pass i + 1 ahead; calls that stop and
+    # resume must see the same streams as one long call.
+    pd = small_random(12, n_be=40, n_gates=60, n_layers=6)
+    tally = _all_nodes(pd)
+    with mc.LayerManager(pd, 2048, tally=tally) as a, mc.LayerManager(pd, 2048, tally=tally) as b:
+        a.pass_(3)
+        b.pass_(1)
+        b.collect_tallies()
+        b.pass_(1)
+        b.wait()
+        b.pass_(1)
+        assert _engine_ones(a) == _engine_ones(b)
+        _, words = orc.run(pd, 2048, first_iter=3, n_iters=1, want_words=True)
+        row = _rows(pd)
+        for v in pd.var_index[::5]:
+            assert np.array_equal(b.read_words(int(v)), words[row[int(v)]])
+
+
+def test_engine_info_reports_double_buffering(gpu):
+    with mc.LayerManager(mc.small_tree(1), 4096) as eng:
+        info = eng.get_info()
+        assert info["event_buffers"] == 2
+        assert info["launches_per_pass"] == info["num_layers"] - 1
+        t = eng.time_passes(2)
+        assert t["launches"] == 2 * (info["num_layers"] - 1) + 1  # one prologue sampler launch
+        assert t["kernel_ms"] > 0
