@@ -49,6 +49,26 @@ def cpu_baseline(pd, target_s: float, max_words: int):
     return words, ones, dt, orc.max_threads()
 
 
+def copy_calibration(n_bytes: int) -> dict:
+    """One device-to-device copy of n_bytes through the HIP runtime's blit
+    kernel: the known-size reference that scales HBM byte counters."""
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so")
+    src, dst = C.c_void_p(), C.c_void_p()
+    for ptr in (src, dst):
+        if hip.hipMalloc(C.byref(ptr), C.c_size_t(n_bytes)) != 0:
+            raise RuntimeError("hipMalloc failed")
+    try:
+        if hip.hipMemset(src, 1, C.c_size_t(n_bytes)) != 0 or hip.hipDeviceSynchronize() != 0:
+            raise RuntimeError("hipMemset failed")
+        if hip.hipMemcpy(dst, src, C.c_size_t(n_bytes), 3) != 0 or hip.hipDeviceSynchronize() != 0:
+            raise RuntimeError("hipMemcpy failed")
+    finally:
+        hip.hipFree(src)
+        hip.hipFree(dst)
+    return {"read": n_bytes, "write": n_bytes}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -58,6 +78,8 @@ def main() -> int:
     ap.add_argument("--tree", choices=["large", "small"], default="large")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="also run a 1 GiB device-to-device copy (reference for HBM counters)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: the engine's sharding over trials is a "
@@ -133,6 +155,8 @@ def main() -> int:
             "sample": f"oracle, same tree, one pass of {words} words ({words * 64} trials), "
                       f"every gate tallied, {dt:.1f} s on {threads} OpenMP threads",
             "parity_bit_exact": [int(x) for x in ones] == gpu_ones}
+    if args.copy_calibration:
+        out["copy_calibration_bytes"] = copy_calibration(1 << 30)
     print(json.dumps(out), flush=True)
     return 0
 

@@ -98,6 +98,10 @@ struct mcs_engine {
 
 namespace {
 
+// Widest AND/OR/XOR family gate evaluated by one task; wider ones are split
+// into partial folds so no task streams far more rows than its neighbours.
+constexpr uint32_t kMaxFanIn = 64;
+
 // Words per workgroup task: 512 (256 lanes x 2 words) times 1, 2 or 4,
 // keeping grids far above the 256 CUs while amortising per-task set-up on
 // long rows.
@@ -158,10 +162,14 @@ mcs_status build_engine(const mcs_pdag *pdag, const mcs_sample_shape *shape,
     const uint64_t reserve = uint64_t(256) << 20;
     const uint64_t one_buf = (uint64_t(plan.n_nodes()) * row_bytes);
     const uint64_t two_buf = one_buf + uint64_t(n_ev) * row_bytes;
-    eng->n_buf = (n_ev > 0 && two_buf + reserve <= free_b) ? 2u : 1u;
-    eng->n_rows = uint64_t(eng->n_buf) * n_ev + plan.n_gates();
+    // Execution schedule (wide gates split into partial folds, each partial
+    // one more row).
+    const mcs::Schedule sch = mcs::build_schedule(plan, kMaxFanIn);
+    const uint64_t aux_bytes = uint64_t(sch.n_aux) * row_bytes;
+    eng->n_buf = (n_ev > 0 && two_buf + aux_bytes + reserve <= free_b) ? 2u : 1u;
+    eng->n_rows = uint64_t(eng->n_buf) * n_ev + plan.n_gates() + uint64_t(sch.n_aux);
 
-    // Launch plan: one launch per gate layer; launch 0 also carries the
+    // Launch plan: one launch per schedule layer; launch 0 also carries the
     // tally-only tasks of tallied basic events.
     std::vector<mcs::GateDesc> gd;
     std::vector<uint32_t> args0, args1;  // argument rows for event buffer 0 / 1
@@ -170,7 +178,7 @@ mcs_status build_engine(const mcs_pdag *pdag, const mcs_sample_shape *shape,
         return node < n_ev ? uint32_t(node + int64_t(buf) * n_ev) : uint32_t(eng->gate_row(node));
     };
     eng->launch_gate_begin.push_back(0);
-    for (int32_t l = 0; l < plan.n_gate_layers(); ++l) {
+    for (size_t l = 0; l < sch.layers.size(); ++l) {
         double cost = 0;
         auto add_gate = [&](uint32_t out_row, uint32_t type, int32_t k, uint32_t flags,
                             const std::vector<int32_t> &pos, const std::vector<int32_t> &neg) {
@@ -188,11 +196,10 @@ mcs_status build_engine(const mcs_pdag *pdag, const mcs_sample_shape *shape,
             gd.push_back(d);
             cost += n_in + ((flags & mcs::kGateNoStore) ? 0 : 1);
         };
-        for (uint32_t g = plan.layer_begin[l]; g < plan.layer_begin[l + 1]; ++g) {
-            const mcs::PlannedGate &pg = plan.gates[g];
-            const int32_t node = n_ev + int32_t(g);
-            add_gate(uint32_t(eng->gate_row(node)), pg.type, pg.k,
-                     tallied[node] ? mcs::kGateTallied : 0u, pg.pos_in, pg.neg_in);
+        for (const mcs::ExecGate &g : sch.layers[l]) {
+            const bool t = g.out < plan.n_nodes() && tallied[g.out];
+            add_gate(uint32_t(eng->gate_row(g.out)), g.type, g.k, t ? mcs::kGateTallied : 0u,
+                     g.pos_in, g.neg_in);
         }
         if (l == 0) {
             for (int32_t e = 0; e < n_ev; ++e)

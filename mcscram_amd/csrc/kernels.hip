@@ -204,44 +204,74 @@ __device__ __forceinline__ void fold_simple(Pair (&r)[K], const uint32_t *__rest
     }
 }
 
-// ATLEAST on one pair per lane: bit-sliced counter (carry-save planes), then
-// the per-bit comparison count >= k8 on the planes. Eight planes hold the
-// reference's 8-bit count, so counts wrap at 256 exactly as it does.
+// ATLEAST: a bit-sliced counter (carry-save planes, plane b = bit b of each
+// trial's count of true inputs) folded over the inputs, then the per-bit
+// comparison count >= k8 on the planes.
+template <typename W, int P>
+__device__ __forceinline__ W atleast_compare(const W (&c)[P], uint32_t k8) {
+    W gt = W(0), eq = ~W(0);
+#pragma unroll
+    for (int b = P - 1; b >= 0; --b) {
+        if ((k8 >> b) & 1u) {
+            eq &= c[b];
+        } else {
+            gt |= eq & c[b];
+            eq &= ~c[b];
+        }
+    }
+    return gt | eq;
+}
+
+// Up to 15 inputs: four planes hold the count exactly; k8 above 15 can never
+// be met (the comparison sees a k bit above the planes as unmet).
+__device__ __forceinline__ Pair atleast_pair4(const uint32_t *__restrict__ a, uint32_t n_in,
+                                              uint32_t n_pos, uint32_t k8,
+                                              const uint64_t *__restrict__ words, uint64_t stride,
+                                              uint64_t s) {
+    uint64_t cx[4] = {0, 0, 0, 0}, cy[4] = {0, 0, 0, 0};
+    for (uint32_t i = 0; i < n_in; ++i) {
+        Pair v = load_pair(words + uint64_t(a[i]) * stride + s);
+        if (i >= n_pos) { v.x = ~v.x; v.y = ~v.y; }
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+            const uint64_t tx = cx[b] & v.x, ty = cy[b] & v.y;
+            cx[b] ^= v.x;
+            cy[b] ^= v.y;
+            v.x = tx;
+            v.y = ty;
+        }
+    }
+    if (k8 > 15u) return {0, 0};
+    return {atleast_compare<uint64_t, 4>(cx, k8), atleast_compare<uint64_t, 4>(cy, k8)};
+}
+
+// Any fan-in: eight planes hold the reference's 8-bit count, which wraps at
+// 256 as its uint8 counter does; one word at a time to bound registers.
+__device__ __forceinline__ uint64_t atleast_word8(const uint32_t *__restrict__ a, uint32_t n_in,
+                                                  uint32_t n_pos, uint32_t k8,
+                                                  const uint64_t *__restrict__ words,
+                                                  uint64_t stride, uint64_t s) {
+    uint64_t c[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    for (uint32_t i = 0; i < n_in; ++i) {
+        uint64_t v = words[uint64_t(a[i]) * stride + s];
+        if (i >= n_pos) v = ~v;
+#pragma unroll
+        for (int b = 0; b < 8; ++b) {
+            const uint64_t t = c[b] & v;
+            c[b] ^= v;
+            v = t;
+        }
+    }
+    return atleast_compare<uint64_t, 8>(c, k8);
+}
+
 __device__ __forceinline__ Pair atleast_pair(const uint32_t *__restrict__ a, uint32_t n_in,
                                              uint32_t n_pos, uint32_t k8,
                                              const uint64_t *__restrict__ words, uint64_t stride,
                                              uint64_t s) {
-    const uint32_t planes = n_in >= 255u ? 8u : 32u - __clz(n_in);
-    Pair c[8];
-#pragma unroll
-    for (int b = 0; b < 8; ++b) c[b] = {0, 0};
-    for (uint32_t i = 0; i < n_in; ++i) {
-        Pair carry = load_pair(words + uint64_t(a[i]) * stride + s);
-        if (i >= n_pos) { carry.x = ~carry.x; carry.y = ~carry.y; }
-#pragma unroll
-        for (uint32_t b = 0; b < 8; ++b) {
-            if (b < planes) {
-                const Pair t = {c[b].x & carry.x, c[b].y & carry.y};
-                c[b].x ^= carry.x;
-                c[b].y ^= carry.y;
-                carry = t;
-            }
-        }
-    }
-    Pair gt = {0, 0}, eq = {~0ull, ~0ull};
-#pragma unroll
-    for (int b = 7; b >= 0; --b) {
-        if ((k8 >> b) & 1u) {
-            eq.x &= c[b].x;
-            eq.y &= c[b].y;
-        } else {
-            gt.x |= eq.x & c[b].x;
-            gt.y |= eq.y & c[b].y;
-            eq.x &= ~c[b].x;
-            eq.y &= ~c[b].y;
-        }
-    }
-    return {gt.x | eq.x, gt.y | eq.y};
+    if (n_in <= 15u) return atleast_pair4(a, n_in, n_pos, k8, words, stride, s);
+    return {atleast_word8(a, n_in, n_pos, k8, words, stride, s),
+            atleast_word8(a, n_in, n_pos, k8, words, stride, s + 1)};
 }
 
 template <int K>

@@ -210,4 +210,57 @@ bool build_plan(const mcs_pdag &pd, Plan &plan, std::string &error) {
     return true;
 }
 
+Schedule build_schedule(const Plan &plan, uint32_t max_fan_in) {
+    Schedule sch;
+    const int32_t n_nodes = plan.n_nodes();
+    std::vector<int32_t> depth(size_t(n_nodes), 0);  // events 0; grows with aux nodes
+    auto place = [&](ExecGate &&g) {
+        int32_t d = 0;
+        for (const auto *v : {&g.pos_in, &g.neg_in})
+            for (int32_t i : *v) d = std::max(d, depth[size_t(i)]);
+        d += 1;
+        if (size_t(g.out) >= depth.size()) depth.resize(size_t(g.out) + 1, 0);
+        depth[size_t(g.out)] = d;
+        if (sch.layers.size() < size_t(d)) sch.layers.resize(size_t(d));
+        sch.layers[size_t(d) - 1].push_back(std::move(g));
+    };
+    const uint32_t cap = std::max<uint32_t>(max_fan_in, 2);
+    for (size_t gi = 0; gi < plan.gates.size(); ++gi) {
+        const PlannedGate &pg = plan.gates[gi];
+        const int32_t node = plan.n_events() + int32_t(gi);
+        const size_t n_in = pg.pos_in.size() + pg.neg_in.size();
+        uint8_t base = 0xFF;
+        if (pg.type == MCS_AND || pg.type == MCS_NAND) base = MCS_AND;
+        if (pg.type == MCS_OR || pg.type == MCS_NOR) base = MCS_OR;
+        if (pg.type == MCS_XOR) base = MCS_XOR;
+        if (base == 0xFF || n_in <= cap) {
+            place(ExecGate{node, pg.type, pg.k, pg.pos_in, pg.neg_in});
+            continue;
+        }
+        // Literals (node, negated), shallow first; stable so equal depths keep
+        // the plan's order.
+        std::vector<std::pair<int32_t, bool>> lits;
+        for (int32_t i : pg.pos_in) lits.push_back({i, false});
+        for (int32_t i : pg.neg_in) lits.push_back({i, true});
+        std::stable_sort(lits.begin(), lits.end(), [&](const auto &a, const auto &b) {
+            return depth[size_t(a.first)] < depth[size_t(b.first)];
+        });
+        while (lits.size() > cap) {
+            std::vector<std::pair<int32_t, bool>> next;
+            for (size_t b = 0; b < lits.size(); b += cap) {
+                ExecGate part{n_nodes + sch.n_aux++, base, 0, {}, {}};
+                for (size_t j = b; j < std::min(lits.size(), b + cap); ++j)
+                    (lits[j].second ? part.neg_in : part.pos_in).push_back(lits[j].first);
+                next.push_back({part.out, false});
+                place(std::move(part));
+            }
+            lits = std::move(next);
+        }
+        ExecGate top{node, pg.type, pg.k, {}, {}};
+        for (const auto &l : lits) (l.second ? top.neg_in : top.pos_in).push_back(l.first);
+        place(std::move(top));
+    }
+    return sch;
+}
+
 }  // namespace mcs

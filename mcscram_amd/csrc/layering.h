@@ -45,4 +45,28 @@ struct Plan {
 // Returns false with a message on malformed input.
 bool build_plan(const mcs_pdag &pdag, Plan &plan, std::string &error);
 
+// One gate evaluation of the execution schedule. out is a node id of the
+// plan, or an id >= plan.n_nodes() for an auxiliary partial fold.
+struct ExecGate {
+    int32_t out;
+    uint8_t type;
+    int32_t k;
+    std::vector<int32_t> pos_in, neg_in;
+};
+
+// Execution schedule: the plan's gates in dependency layers, with every
+// AND/OR/XOR/NAND/NOR gate of more than max_fan_in inputs split into a tree
+// of partial folds of the same bitwise operation (AND for NAND, OR for NOR),
+// the gate itself folding the partials. Bitwise AND, OR and XOR are
+// associative and commutative, so every word is unchanged; only the launch
+// that evaluates each piece moves. Inputs are grouped in order of depth so
+// partials over shallow inputs run in early layers. Event order and tallies
+// are those of the plan.
+struct Schedule {
+    std::vector<std::vector<ExecGate>> layers;  // layers[0] = first gate layer
+    int32_t n_aux = 0;
+};
+
+Schedule build_schedule(const Plan &plan, uint32_t max_fan_in);
+
 }  // namespace mcs

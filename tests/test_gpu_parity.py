@@ -190,3 +190,23 @@ def test_engine_info_reports_double_buffering(gpu):
         t = eng.time_passes(2)
         assert t["launches"] == 2 * (info["num_layers"] - 1) + 1  # one prologue sampler launch
         assert t["kernel_ms"] > 0
+
+
+@pytest.mark.parametrize("conn", [mc.pdag.AND, mc.pdag.OR, mc.pdag.XOR, mc.pdag.NAND, mc.pdag.NOR])
+def test_wide_gates_split_into_partial_folds(gpu, conn):
+    # Gates wider than one task's limit are evaluated as trees of partial
+    # folds; inputs mix basic events, gates of several depths and negations.
+    n_be = 400
+    vs = [(2 + i, 0.98 if conn in (mc.pdag.AND, mc.pdag.NAND) else 0.002) for i in range(n_be)]
+    gates, nxt = [], 2 + n_be
+    for j in range(30):  # depth-1 and depth-2 helpers
+        gates.append((nxt, OR, 0, [2 + (7 * j) % n_be, 2 + (11 * j + 3) % n_be]))
+        nxt += 1
+    for j in range(10):
+        gates.append((nxt, mc.pdag.AND, 0, [nxt - 30 + j, -(2 + j)]))
+        nxt += 1
+    wide = [2 + i if i % 9 else -(2 + i) for i in range(n_be)] + list(range(2 + n_be, nxt))
+    gates.append((nxt, conn, 0, wide))
+    gates.append((nxt + 1, OR, 0, [nxt, 2]))
+    pd = Pdag.from_lists(vs, gates, nxt + 1)
+    _check_tallies(pd, 4096, n_pass=2)
