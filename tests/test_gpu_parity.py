@@ -186,9 +186,11 @@ def test_engine_info_reports_double_buffering(gpu):
     with mc.LayerManager(mc.small_tree(1), 4096) as eng:
         info = eng.get_info()
         assert info["event_buffers"] == 2
-        assert info["launches_per_pass"] == info["num_layers"] - 1
+        # One launch per schedule layer: at least the gate layers, plus one for
+        # each level of partial folds a wide gate adds.
+        assert info["launches_per_pass"] >= info["num_layers"] - 1
         t = eng.time_passes(2)
-        assert t["launches"] == 2 * (info["num_layers"] - 1) + 1  # one prologue sampler launch
+        assert t["launches"] == 2 * info["launches_per_pass"] + 1  # one prologue sampler launch
         assert t["kernel_ms"] > 0
 
 
