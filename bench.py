@@ -49,6 +49,22 @@ def cpu_baseline(pd, target_s: float, max_words: int):
     return words, ones, dt, orc.max_threads()
 
 
+def measured_traffic(tree: str, n_bp: int):
+    """HBM bytes per launch of pass_layer from the committed rocprofv3
+    FETCH_SIZE / WRITE_SIZE summary of this workload (scaled by the copy
+    calibration recorded there), or None when no summary matches."""
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
+                        f"pass_layer_{tree}_tree_hbm_counters.json")
+    try:
+        with open(path) as f:
+            s = json.load(f)
+    except OSError:
+        return None
+    if s.get("words_per_pass") != n_bp or s.get("tree") != tree:
+        return None
+    return s["traffic_bytes_per_launch"]
+
+
 def copy_calibration(n_bytes: int) -> dict:
     """One device-to-device copy of n_bytes through the HIP runtime's blit
     kernel: the known-size reference that scales HBM byte counters."""
@@ -133,7 +149,8 @@ def main() -> int:
         "node_evals_per_s": (pd.n_vars + pd.n_gates) * trials / wall,
         "roofline": {"kernel": "pass_layer", "bound": "hbm", "achieved": achieved,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": None, "algorithmic_bytes_per_launch": bytes_per_launch,
+                     "traffic": measured_traffic(args.tree, n_bp),
+                     "algorithmic_bytes_per_launch": bytes_per_launch,
                      "avg_launch_ms": launch_s * 1e3, "launches_per_pass": launches_per_pass},
         "pass_roofline": {"bound": "hbm", "algorithmic_bytes_per_pass": ab["pass"],
                           "achieved": pass_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
