@@ -85,6 +85,79 @@ def copy_calibration(n_bytes: int) -> dict:
     return {"read": n_bytes, "write": n_bytes}
 
 
+def bench_fused(args, mc) -> int:
+    """Batched uneven trees (or the small tree alone) in the persistent
+    whole-tree-in-LDS kernel; one step = one pass over every tree."""
+    from oracle import binding as orc
+    if args.workload == "batched":
+        trees = mc.batched_trees(seed=3, n_trees=args.trees)
+        words = 1 << 20  # 2^26 trials per tree per pass
+        name = f"batched plain MC, {args.trees} uneven trees (50-5000 gates), every gate tallied"
+    else:
+        trees = [mc.small_tree(1)]
+        words = 16 * 1024  # 2^20 trials
+        name = "plain MC, small layered tree, every gate tallied, fused kernel"
+    tallies = [[int(g) for g in t.gate_index] for t in trees]
+    cfg = tuned_batch_config(args.workload)
+    b = mc.Batch(trees, words, tallies, **cfg)
+    info = b.info()
+    if args.warmup:
+        b.time_passes(args.warmup)
+    b.wait()
+    t0 = time.perf_counter()
+    dev_ms = b.time_passes(args.steps)
+    wall = time.perf_counter() - t0
+    n_g = sum(t.n_gates for t in trees)
+    n_be = sum(t.n_vars for t in trees)
+    trials = args.steps * words * 64
+    value = n_g * trials / wall
+    # VALU roofline of the sampler: Philox words at the measured issue cost.
+    out = {
+        "metric": "bit-level gate-evals/s", "value": value, "unit": "gate-evals/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": wall / args.steps * 1e3, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "u64", "data": "synthetic",
+        "config": {"workload": name, "trees": len(trees), "basic_events": n_be, "gates": n_g,
+                   "words_per_pass": words, "trials_per_tree_per_pass": words * 64,
+                   "batch_config": cfg, "grid": info["grid"], "lds_bytes": info["lds_bytes"],
+                   "work_items": info["num_items"]},
+        "node_evals_per_s": (n_g + n_be) * trials / wall,
+        "device_ms_per_pass": dev_ms / args.steps,
+        "philox_words_per_s": n_be * words * args.steps / wall,
+        "hbm_bytes_per_pass": "tallies only (trees live in LDS)",
+    }
+    if not args.no_cpu:
+        # CPU oracle on a fixed subset of the trees, bit-exact check beside it.
+        sub = list(range(0, len(trees), max(1, len(trees) // 8)))[:8]
+        sw = 1 << 10 if args.workload == "batched" else words
+        t = time.perf_counter()
+        cpu = [orc.run(trees[i], sw, tally=tallies[i])[0] for i in sub]
+        dt = time.perf_counter() - t
+        with mc.Batch([trees[i] for i in sub], sw, [tallies[i] for i in sub], **cfg) as small:
+            small.pass_(1)
+            ok = all([x.num_one_bits for x in small.collect(j)] == [int(v) for v in cpu[j]]
+                     for j in range(len(sub)))
+        out["cpu_baseline"] = {
+            "value": sum(trees[i].n_gates for i in sub) * sw * 64 / dt, "unit": "gate-evals/s",
+            "cores": orc.max_threads(), "kind": "port",
+            "sample": f"oracle, trees {sub}, one pass of {sw} words each, {dt:.1f} s",
+            "parity_bit_exact": ok}
+    b.close()
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+def tuned_batch_config(workload: str) -> dict:
+    """Batch configuration chosen by the autotuner for this workload, if one
+    was recorded (mcscram_amd/tuned.json), else the defaults."""
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "mcscram_amd", "tuned.json")
+    try:
+        with open(path) as f:
+            return dict(json.load(f).get(workload, {}).get("config", {}))
+    except OSError:
+        return {}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -92,6 +165,11 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--words", type=int, default=1 << 18, help="uint64 words per pass")
     ap.add_argument("--tree", choices=["large", "small"], default="large")
+    ap.add_argument("--workload", choices=["layered", "batched", "small-fused"], default="layered",
+                    help="layered: the large tree through the layered engine (default); "
+                         "batched: 256 uneven trees, 2^26 trials each, one persistent launch "
+                         "per pass; small-fused: the small tree, 2^20 trials, fused kernel")
+    ap.add_argument("--trees", type=int, default=256)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -103,6 +181,8 @@ def main() -> int:
         return 2
 
     import mcscram_amd as mc
+    if args.workload != "layered":
+        return bench_fused(args, mc)
     pd = mc.large_tree() if args.tree == "large" else mc.small_tree(1)
     tally = [int(g) for g in pd.gate_index]
     n_bp = args.words

@@ -149,6 +149,12 @@ mcs_status mcs_engine_pass_wait_collect(mcs_engine *engine, uint32_t total_passe
  * tallied nodes; order = first appearance in tally_index. Waits first. */
 mcs_status mcs_engine_collect(mcs_engine *engine, mcs_tally *out, int32_t n_out);
 
+/* Point estimates of a tally from its raw counters: mean, standard error
+ * and the normal-approximation 95 % / 99 % intervals, clamped to [0, 1]
+ * (stats::populate_point_estimates, src/mc/stats/ci_utils.h:167-201).
+ * Reads num_one_bits and total_bits, writes mean, std_err and ci. */
+void mcs_tally_point_estimates(mcs_tally *t);
+
 /* Engine geometry and counters. */
 mcs_status mcs_engine_get_info(mcs_engine *engine, mcs_engine_info *out);
 
@@ -229,6 +235,58 @@ mcs_status mcs_probability_analysis(const mcs_pdag *pdag, const mcs_settings *se
                                     const int32_t *observe, int32_t n_observe,
                                     mcs_tally *out, int32_t n_out, double *p_total,
                                     uint32_t *passes_run);
+
+
+/*
+ * Batches of independent trees evaluated by one persistent launch per pass.
+ * No counterpart in the reference, which builds one layer_manager per PDAG
+ * (src/mc/queue/layer_manager.tpp:344-375); each tree of a batch gives the
+ * same tallies as its own engine (same Philox key, counters and sample
+ * shape). A tree is evaluated whole in LDS: it must fit at one word per node
+ * (about 20,000 nodes at the default budget); larger trees use
+ * mcs_engine_create. Every tree advances by one iteration per pass.
+ */
+typedef struct mcs_batch mcs_batch;
+
+typedef struct mcs_batch_config {
+    uint32_t lds_bytes;       /* LDS per workgroup for a tile (0 = 64 KiB) */
+    uint32_t max_tile_words;  /* words per tile, power of two <= 64 (0 = 64) */
+    uint32_t item_words;      /* words per work item (0 = 4096) */
+    uint32_t use_graph;       /* 1: replay each pass as a captured hipGraph */
+    uint32_t key0, key1;      /* Philox key (the reference key by default) */
+} mcs_batch_config;
+
+typedef struct mcs_batch_info {
+    int32_t num_trees;
+    uint32_t num_items;       /* work items per pass */
+    uint32_t grid;            /* persistent workgroups per launch */
+    uint32_t lds_bytes;       /* LDS per workgroup */
+    uint32_t iteration;       /* passes run */
+    uint64_t device_bytes;
+} mcs_batch_info;
+
+/* Fill a config with the defaults. */
+void mcs_batch_config_default(mcs_batch_config *cfg);
+
+/* pdags[n_trees] with shapes[n_trees]; tallies of tree t are
+ * tally_index[tally_offsets[t] .. tally_offsets[t+1]) (an empty range tallies
+ * the root). cfg may be null (defaults). */
+mcs_status mcs_batch_create(const mcs_pdag *pdags, int32_t n_trees,
+                            const mcs_sample_shape *shapes, const int32_t *tally_offsets,
+                            const int32_t *tally_index, const mcs_batch_config *cfg,
+                            mcs_batch **out);
+mcs_status mcs_batch_destroy(mcs_batch *batch);
+/* Enqueue n_passes passes (one launch each) without waiting. */
+mcs_status mcs_batch_pass(mcs_batch *batch, uint32_t n_passes);
+mcs_status mcs_batch_wait(mcs_batch *batch);
+/* Tallies of one tree, in the order of its tally list (waits first). */
+mcs_status mcs_batch_collect(mcs_batch *batch, int32_t tree, mcs_tally *out, int32_t n_out);
+mcs_status mcs_batch_get_info(mcs_batch *batch, mcs_batch_info *out);
+/* Tile width (words) chosen for a tree. */
+mcs_status mcs_batch_tile_words(mcs_batch *batch, int32_t tree, uint32_t *out);
+/* Run n_passes with HIP events around each pass on the batch's stream;
+ * kernel_ms receives the summed device time. Measurement hook. */
+mcs_status mcs_batch_time_passes(mcs_batch *batch, uint32_t n_passes, double *kernel_ms);
 
 #ifdef __cplusplus
 }

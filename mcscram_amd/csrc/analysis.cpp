@@ -11,9 +11,9 @@
 
 #include "mcscram_amd.h"
 
-namespace mcs {
+#include "status.h"
 
-void set_error(const char *msg);  // engine.cpp
+namespace mcs {
 
 // Two-sided normal quantile z with P(|Z| <= z) = confidence, by Acklam's
 // rational approximation of the inverse normal CDF (P. J. Acklam, "An

@@ -23,29 +23,21 @@
 #include "kernels.h"
 #include "layering.h"
 #include "mcscram_amd.h"
+#include "status.h"
 
 namespace {
-
 thread_local std::string g_error;
+}  // namespace
 
+namespace mcs {
 mcs_status fail(mcs_status code, std::string msg) {
     g_error = std::move(msg);
     return code;
 }
-
-#define MCS_HIP(expr)                                                                   \
-    do {                                                                                \
-        const hipError_t e_ = (expr);                                                   \
-        if (e_ != hipSuccess)                                                           \
-            return fail(e_ == hipErrorOutOfMemory ? MCS_ERR_OUT_OF_MEMORY : MCS_ERR_DEVICE, \
-                        std::string(#expr " failed: ") + hipGetErrorString(e_));        \
-    } while (0)
-
-}  // namespace
-
-namespace mcs {
 void set_error(const char *msg) { g_error = msg; }
 }  // namespace mcs
+
+using mcs::fail;
 
 struct mcs_engine {
     int device = 0;
@@ -243,17 +235,8 @@ mcs_status build_engine(const mcs_pdag *pdag, const mcs_sample_shape *shape,
 
     // Event descriptors.
     std::vector<mcs::EventDesc> ev(n_ev);
-    for (int32_t e = 0; e < n_ev; ++e) {
-        const double p = plan.event_p[e];
-        const uint64_t thr = static_cast<uint64_t>(p * 4294967296.0);
-        uint32_t mode = mcs::kEventDraw;
-        if (thr == 0) mode = mcs::kEventNever;
-        if (thr >= (uint64_t(1) << 32)) mode = mcs::kEventAlways;
-        ev[e].ctr0 = uint32_t(plan.event_pdag_index[e]) + 1u;
-        ev[e].ctr1 = uint32_t(e) + 1u;
-        ev[e].threshold = uint32_t(std::min<uint64_t>(thr, 0xFFFFFFFFull));
-        ev[e].mode = mode;
-    }
+    for (int32_t e = 0; e < n_ev; ++e)
+        ev[e] = mcs::make_event_desc(plan.event_pdag_index[e], uint32_t(e), plan.event_p[e]);
 
     // Device allocation.
     MCS_HIP(hipStreamCreateWithFlags(&eng->stream, hipStreamNonBlocking));
@@ -350,7 +333,7 @@ mcs_status enqueue_passes(mcs_engine *e, uint32_t n, LaunchHook *hook = nullptr)
     return MCS_OK;
 }
 
-void fill_estimates(mcs_tally &t) {
+void fill_estimates_impl(mcs_tally &t) {
     if (t.total_bits == 0) {
         t.mean = t.std_err = 0.0;
         for (double &c : t.ci) c = 0.0;
@@ -382,7 +365,7 @@ mcs_status collect(mcs_engine *e, mcs_tally *out, int32_t n_out) {
         r.num_one_bits = ones[e->tally_slot(e->tally_nodes[t])];
         // total = iteration * N_bp * 64 (src/mc/kernel/tally.h:345-352).
         r.total_bits = uint64_t(e->iteration) * e->geo.n_bp * 64ull;
-        fill_estimates(r);
+        mcs_tally_point_estimates(&r);
     }
     return MCS_OK;
 }
@@ -410,6 +393,10 @@ struct EventTimer : LaunchHook {
 extern "C" {
 
 const char *mcs_last_error(void) { return g_error.c_str(); }
+
+void mcs_tally_point_estimates(mcs_tally *t) {
+    if (t) fill_estimates_impl(*t);
+}
 
 mcs_status mcs_engine_create(const mcs_pdag *pdag, const mcs_sample_shape *shape,
                              const int32_t *tally_index, int32_t n_tally, mcs_engine **out) {

@@ -1,0 +1,181 @@
+// Persistent whole-tree-in-LDS kernel for batches of small trees (CDNA4).
+//
+// Workgroups of 256 lanes pull work items (tree, tiles) from a queue in HBM.
+// For each tile of W words a workgroup samples every basic event of the tree
+// into LDS, then evaluates the gate layers from LDS with a workgroup barrier
+// between layers. Lane l of a step owns word (l mod W) of node (l / W), so
+// the W words of one node are W neighbouring lanes of one wave: their
+// popcounts are summed with lane shuffles and added to the node's LDS
+// counter by one lane. Counters are flushed to HBM with one 64-bit atomic
+// per node per item.
+//
+// Gate semantics are those of the layered kernel (src/mc/kernel/gate.h:
+// 262-264, 299-362; ATLEAST 506-555 with the 8-bit count); the sampler is
+// the same Bernoulli word (src/mc/kernel/basic_event.h:112-142).
+#include <hip/hip_runtime.h>
+
+#include "device_ops.h"
+#include "fused.h"
+
+namespace mcs {
+namespace {
+
+constexpr uint32_t kFusedLanes = 256;
+
+// Sum of v over the aligned group of w lanes (w a power of two <= 64).
+__device__ __forceinline__ uint32_t group_sum(uint32_t v, uint32_t w) {
+    for (uint32_t off = 1; off < w; off <<= 1) v += __shfl_xor(v, int(off), 64);
+    return v;
+}
+
+__device__ __forceinline__ uint64_t lds_word(const uint64_t *lds, uint32_t node, uint32_t log2_w,
+                                             uint32_t w) {
+    return lds[(node << log2_w) + w];
+}
+
+__device__ __forceinline__ uint64_t eval_gate_word(const FusedGate &g,
+                                                   const uint32_t *__restrict__ a,
+                                                   const uint64_t *lds, uint32_t log2_w,
+                                                   uint32_t w) {
+    const uint32_t n_in = g.counts & 0xFFFFu;
+    const uint32_t n_pos = g.counts >> 16;
+    const uint32_t type = g.type_k_flags & 0xFFu;
+    const uint32_t k8 = (g.type_k_flags >> 8) & 0xFFu;
+    uint64_t r;
+    if (type == 2u) {  // ATLEAST: carry-save planes of the per-trial count
+        uint64_t c[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+        for (uint32_t i = 0; i < n_in; ++i) {
+            uint64_t v = lds_word(lds, a[i], log2_w, w);
+            if (i >= n_pos) v = ~v;
+#pragma unroll
+            for (int b = 0; b < 8; ++b) {
+                const uint64_t t = c[b] & v;
+                c[b] ^= v;
+                v = t;
+            }
+        }
+        r = atleast_compare<uint64_t, 8>(c, k8);
+    } else if (type == 4u || type == 7u) {  // NOT, NULL: the last input
+        r = 0;
+        if (n_in) {
+            r = lds_word(lds, a[n_in - 1], log2_w, w);
+            if (n_in - 1 >= n_pos) r = ~r;
+        }
+    } else {
+        const bool and_like = (type == 0u || type == 5u);
+        r = and_like ? ~0ull : 0ull;
+        for (uint32_t i = 0; i < n_in; ++i) {
+            uint64_t v = lds_word(lds, a[i], log2_w, w);
+            if (i >= n_pos) v = ~v;
+            if (and_like)
+                r &= v;
+            else if (type == 3u)
+                r ^= v;
+            else
+                r |= v;
+        }
+    }
+    if (type == 4u || type == 5u || type == 6u) r = ~r;  // NOT, NAND, NOR
+    return r;
+}
+
+}  // namespace
+
+__global__ void fused_begin_pass(uint32_t *state) {
+    if (threadIdx.x == 0) {
+        state[0] = 0;
+        state[1] += 1;
+    }
+}
+
+__global__ __launch_bounds__(256) void fused_trees(FusedLaunch L) {
+    extern __shared__ uint64_t lds[];
+    __shared__ uint32_t s_item;
+    const uint32_t tid = threadIdx.x;
+    const uint32_t iteration = L.state[1];
+    const uint32_t c3base = (iteration + 1u) << 6;
+    for (;;) {
+        if (tid == 0) s_item = atomicAdd(L.state, 1u);
+        __syncthreads();
+        const uint32_t item = s_item;
+        __syncthreads();
+        if (item >= L.n_items) break;
+        const WorkItem wi = L.items[item];
+        const FusedTree T = L.trees[wi.tree];
+        const uint32_t log2_w = T.log2_w;
+        const uint32_t W = 1u << log2_w;
+        uint32_t *cnt = reinterpret_cast<uint32_t *>(lds + (uint64_t(T.n_nodes) << log2_w));
+        for (uint32_t n = tid; n < T.n_nodes; n += kFusedLanes) cnt[n] = 0;
+        __syncthreads();
+        const EventDesc *events = L.events + T.ev_begin;
+        const FusedGate *gates = L.gates + T.gate_begin;
+        const uint32_t *layer_off = L.layer_off + T.layer_begin;
+        for (uint32_t tile = 0; tile < wi.n_tiles; ++tile) {
+            const uint64_t base = uint64_t(wi.first_tile + tile) << log2_w;
+            // Basic events of the tile.
+            const uint32_t n_ev_items = T.n_ev << log2_w;
+            for (uint32_t idx = tid; idx < n_ev_items; idx += kFusedLanes) {
+                const uint32_t e = idx >> log2_w, w = idx & (W - 1u);
+                const uint64_t s = base + w;
+                const EventDesc d = events[e];
+                const uint32_t mode = d.mode & 3u;
+                uint64_t word = 0;
+                if (s < T.n_bp) {
+                    if (mode == kEventDraw)
+                        word = bernoulli_word(d.ctr0, d.ctr1, uint32_t(s) + 1u, c3base, L.key0,
+                                              L.key1, d.threshold);
+                    else if (mode == kEventAlways)
+                        word = ~0ull;
+                }
+                lds[idx] = word;
+                if (d.mode & kEventTallied) {
+                    const uint32_t pc = group_sum(uint32_t(__popcll(word)), W);
+                    if (w == 0 && pc) atomicAdd(cnt + e, pc);
+                }
+            }
+            __syncthreads();
+            // Gate layers, each from LDS only.
+            for (uint32_t l = 0; l < T.n_layers; ++l) {
+                const uint32_t g0 = layer_off[l];
+                const uint32_t n_items = (layer_off[l + 1] - g0) << log2_w;
+                for (uint32_t idx = tid; idx < n_items; idx += kFusedLanes) {
+                    const uint32_t g = g0 + (idx >> log2_w), w = idx & (W - 1u);
+                    const FusedGate gd = gates[g];
+                    const uint64_t r = eval_gate_word(gd, L.args + gd.arg_begin, lds, log2_w, w);
+                    const uint32_t node = T.n_ev + g;
+                    lds[(node << log2_w) + w] = r;
+                    if (gd.type_k_flags & kGateTallied) {
+                        const uint32_t pc =
+                            group_sum(base + w < T.n_bp ? uint32_t(__popcll(r)) : 0u, W);
+                        if (w == 0 && pc) atomicAdd(cnt + node, pc);
+                    }
+                }
+                __syncthreads();
+            }
+        }
+        for (uint32_t n = tid; n < T.n_nodes; n += kFusedLanes) {
+            const uint32_t c = cnt[n];
+            if (c) atomicAdd(L.ones + T.ones_begin + n, (unsigned long long)c);
+        }
+        __syncthreads();
+    }
+}
+
+hipError_t fused_max_workgroups_per_cu(uint32_t lds_bytes, int *out) {
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(out, fused_trees, int(kFusedLanes),
+                                                        size_t(lds_bytes));
+}
+
+hipError_t launch_fused_begin_pass(hipStream_t stream, uint32_t *state) {
+    hipLaunchKernelGGL(fused_begin_pass, dim3(1), dim3(64), 0, stream, state);
+    return hipGetLastError();
+}
+
+hipError_t launch_fused(hipStream_t stream, const FusedLaunch &l, uint32_t grid,
+                        uint32_t lds_bytes) {
+    if (grid == 0) return hipSuccess;
+    hipLaunchKernelGGL(fused_trees, dim3(grid), dim3(kFusedLanes), lds_bytes, stream, l);
+    return hipGetLastError();
+}
+
+}  // namespace mcs

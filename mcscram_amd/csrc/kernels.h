@@ -28,8 +28,20 @@ struct EventDesc {     // 16 bytes, one per basic event, event_idx order
     uint32_t ctr0;     // PDAG index + 1       (Philox counter word 0)
     uint32_t ctr1;     // event_idx + 1         (Philox counter word 1)
     uint32_t threshold;// p * 2^32, truncated, when mode == kEventDraw
-    uint32_t mode;
+    uint32_t mode;     // kEventDraw / kEventNever / kEventAlways in bits 0-1
 };
+
+// Descriptor of one basic event: threshold uint64(p * 2^32), truncated
+// (src/mc/event/node.h:371-376); 0 never draws true and 2^32 always does.
+inline EventDesc make_event_desc(int32_t pdag_index, uint32_t event_idx, double p) {
+    const uint64_t thr = static_cast<uint64_t>(p * 4294967296.0);
+    EventDesc d{};
+    d.ctr0 = uint32_t(pdag_index) + 1u;
+    d.ctr1 = event_idx + 1u;
+    d.threshold = uint32_t(thr < 0xFFFFFFFFull ? thr : 0xFFFFFFFFull);
+    d.mode = thr == 0 ? kEventNever : (thr >= (uint64_t(1) << 32) ? kEventAlways : kEventDraw);
+    return d;
+}
 
 // Gate flags (GateDesc::type_k_flags bits 16..).
 constexpr uint32_t kGateTallied = 1u << 16;  // popcount into ones[out_row]

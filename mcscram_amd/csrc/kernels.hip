@@ -19,6 +19,7 @@
 //   ones += popcount(word)             src/mc/kernel/tally.h:322-360
 #include <hip/hip_runtime.h>
 
+#include "device_ops.h"
 #include "kernels.h"
 
 namespace mcs {
@@ -26,99 +27,6 @@ namespace {
 
 constexpr uint32_t kLanes = 256;
 constexpr uint32_t kXcds = 8;  // workgroups are dealt to the 8 XCDs round-robin
-
-__device__ __forceinline__ void mul_wide(uint32_t a, uint32_t m, uint32_t &hi, uint32_t &lo) {
-    const uint64_t p = uint64_t(a) * uint64_t(m);
-    hi = uint32_t(p >> 32);
-    lo = uint32_t(p);
-}
-
-// a ^ b ^ c in one VALU op (v_bitop3_b32, truth table 0x96).
-__device__ __forceinline__ uint32_t xor3(uint32_t a, uint32_t b, uint32_t c) {
-    return __builtin_amdgcn_bitop3_b32(a, b, c, 0x96);
-}
-
-__device__ __forceinline__ void philox_round(uint32_t &x0, uint32_t &x1, uint32_t &x2,
-                                             uint32_t &x3, uint32_t ka, uint32_t kb) {
-    uint32_t ha, la, hb, lb;
-    mul_wide(x0, kPhiloxM0, ha, la);
-    mul_wide(x2, kPhiloxM1, hb, lb);
-    x0 = xor3(hb, x1, ka);
-    x1 = lb;
-    x2 = xor3(ha, x3, kb);
-    x3 = la;
-}
-
-// 64 Bernoulli draws for one word: Philox4x32-10 on counters
-// (c0, c1, c2, c3base + g), g = 0..15, bit 4g+j = (y_j < thr).
-//
-// The 16 counters differ only in word 3, which enters round 1 through an
-// XOR alone. So M1*c2 (round 1), M0*x0 (round 2) and M1*x2 (round 3) are the
-// same for all 16 generations and are computed once per word, and the
-// generation-dependent half of round 2 depends only on per-event values
-// (c0, c3base, g), which are wave-uniform here and go to the scalar unit.
-//
-// Packing: the four draws of a generation hit only if their minimum is below
-// the threshold; when no lane of the wave hits (most generations at the
-// small probabilities of fault trees) the compares and shifts are skipped by
-// one wave-uniform branch.
-__device__ __forceinline__ uint64_t bernoulli_word(uint32_t c0, uint32_t c1, uint32_t c2,
-                                                   uint32_t c3base, uint32_t k0, uint32_t k1,
-                                                   uint32_t thr) {
-    // Round 1: (c0, c1, c2, c3) with key (k0, k1).
-    uint32_t hA, lA, hB, lB;
-    mul_wide(c2, kPhiloxM1, hA, lA);  // per word
-    mul_wide(c0, kPhiloxM0, hB, lB);  // per event
-    const uint32_t a0 = hA ^ (c1 ^ k0);
-    const uint32_t a1 = lA;
-    const uint32_t a2base = hB ^ k1;  // a2 = a2base ^ c3
-    const uint32_t a3 = lB;
-    // Round 2: the a0 half is shared by all generations.
-    const uint32_t k0r1 = k0 + kPhiloxW0, k1r1 = k1 + kPhiloxW1;
-    uint32_t hC, lC;
-    mul_wide(a0, kPhiloxM0, hC, lC);
-    const uint32_t b2 = hC ^ (a3 ^ k1r1);
-    const uint32_t b3 = lC;
-    // Round 3: the b2 half is shared by all generations.
-    const uint32_t k0r2 = k0 + 2u * kPhiloxW0, k1r2 = k1 + 2u * kPhiloxW1;
-    uint32_t hD, lD;
-    mul_wide(b2, kPhiloxM1, hD, lD);
-
-    uint32_t lo_bits = 0, hi_bits = 0;
-#pragma unroll
-    for (uint32_t g = 0; g < 16; ++g) {
-        const uint32_t a2 = a2base ^ (c3base + g);  // per event
-        uint32_t hE, lE;
-        mul_wide(a2, kPhiloxM1, hE, lE);  // per event
-        const uint32_t b0 = a1 ^ (hE ^ k0r1);
-        const uint32_t b1 = lE;
-        uint32_t hF, lF;
-        mul_wide(b0, kPhiloxM0, hF, lF);
-        uint32_t x0 = hD ^ (b1 ^ k0r2);
-        uint32_t x1 = lD;
-        uint32_t x2 = xor3(hF, b3, k1r2);
-        uint32_t x3 = lF;
-#pragma unroll
-        for (uint32_t r = 3; r < 10; ++r)
-            philox_round(x0, x1, x2, x3, k0 + r * kPhiloxW0, k1 + r * kPhiloxW1);
-        const uint32_t m = min(min(x0, x1), min(x2, x3));
-        if (__builtin_amdgcn_ballot_w64(m < thr) != 0) {
-            const uint32_t nib = uint32_t(x0 < thr) | uint32_t(x1 < thr) << 1 |
-                                 uint32_t(x2 < thr) << 2 | uint32_t(x3 < thr) << 3;
-            if (g < 8)
-                lo_bits |= nib << (4u * g);
-            else
-                hi_bits |= nib << (4u * (g - 8u));
-        }
-    }
-    return uint64_t(hi_bits) << 32 | lo_bits;
-}
-
-__device__ __forceinline__ uint32_t wave_sum(uint32_t v) {
-#pragma unroll
-    for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
-    return v;
-}
 
 __device__ __forceinline__ uint32_t live_popcount(uint64_t w0, uint64_t w1, uint64_t s,
                                                   uint64_t n_bp) {
@@ -202,24 +110,6 @@ __device__ __forceinline__ void fold_simple(Pair (&r)[K], const uint32_t *__rest
             apply(r[k], v);
         }
     }
-}
-
-// ATLEAST: a bit-sliced counter (carry-save planes, plane b = bit b of each
-// trial's count of true inputs) folded over the inputs, then the per-bit
-// comparison count >= k8 on the planes.
-template <typename W, int P>
-__device__ __forceinline__ W atleast_compare(const W (&c)[P], uint32_t k8) {
-    W gt = W(0), eq = ~W(0);
-#pragma unroll
-    for (int b = P - 1; b >= 0; --b) {
-        if ((k8 >> b) & 1u) {
-            eq &= c[b];
-        } else {
-            gt |= eq & c[b];
-            eq &= ~c[b];
-        }
-    }
-    return gt | eq;
 }
 
 // Up to 15 inputs: four planes hold the count exactly; k8 above 15 can never

@@ -54,6 +54,17 @@ class _Info(C.Structure):
                 ("launches_per_pass", C.c_uint32)]
 
 
+class _BatchConfig(C.Structure):
+    _fields_ = [("lds_bytes", C.c_uint32), ("max_tile_words", C.c_uint32),
+                ("item_words", C.c_uint32), ("use_graph", C.c_uint32),
+                ("key0", C.c_uint32), ("key1", C.c_uint32)]
+
+
+class _BatchInfo(C.Structure):
+    _fields_ = [("num_trees", C.c_int32), ("num_items", C.c_uint32), ("grid", C.c_uint32),
+                ("lds_bytes", C.c_uint32), ("iteration", C.c_uint32), ("device_bytes", C.c_uint64)]
+
+
 class _Settings(C.Structure):
     _fields_ = [("num_trials", C.c_uint64), ("burn_in_trials", C.c_uint64),
                 ("ci_confidence", C.c_double), ("ci_rel_margin", C.c_double),
@@ -92,6 +103,14 @@ def load_library():
         "mcs_pdag_event_order": [vp, vp, i32, vp],
         "mcs_pdag_gate_depths": [vp, vp, i32],
         "mcs_probability_analysis": [vp, vp, vp, vp, i32, vp, i32, vp, vp],
+        "mcs_batch_create": [vp, i32, vp, vp, vp, vp, vp],
+        "mcs_batch_destroy": [vp],
+        "mcs_batch_pass": [vp, u32],
+        "mcs_batch_wait": [vp],
+        "mcs_batch_collect": [vp, i32, vp, i32],
+        "mcs_batch_get_info": [vp, vp],
+        "mcs_batch_tile_words": [vp, i32, vp],
+        "mcs_batch_time_passes": [vp, u32, vp],
     }
     for name, args in sigs.items():
         fn = getattr(L, name)
@@ -99,6 +118,10 @@ def load_library():
         fn.restype = i32
     L.mcs_settings_default.argtypes = [vp]
     L.mcs_settings_default.restype = None
+    L.mcs_batch_config_default.argtypes = [vp]
+    L.mcs_batch_config_default.restype = None
+    L.mcs_tally_point_estimates.argtypes = [vp]
+    L.mcs_tally_point_estimates.restype = None
     L.mcs_normal_quantile_two_sided.argtypes = [C.c_double]
     L.mcs_normal_quantile_two_sided.restype = C.c_double
     _LIB = L
@@ -144,6 +167,12 @@ class Tally:
     @staticmethod
     def _from(t: _Tally) -> "Tally":
         return Tally(t.index, int(t.num_one_bits), int(t.total_bits), t.mean, t.std_err, tuple(t.ci))
+
+
+def _is_shape(s) -> bool:
+    ints = (int, np.integer)
+    return isinstance(s, ints) or (isinstance(s, tuple) and len(s) == 2 and
+                                   all(isinstance(x, ints) for x in s))
 
 
 def _shape(n_bp: int | tuple) -> _Shape:
@@ -299,3 +328,90 @@ def pdag_gate_depths(pdag: Pdag) -> np.ndarray:
 
 def normal_quantile_two_sided(confidence: float) -> float:
     return load_library().mcs_normal_quantile_two_sided(confidence)
+
+
+class Batch:
+    """Independent trees evaluated whole in LDS by one persistent launch per
+    pass (no reference counterpart; each tree's tallies equal those of its
+    own LayerManager with the same shape).
+
+    ``shapes``: one shape per tree (int words or (batch, bitpacks)), or one
+    shape for all. ``tallies``: one list of PDAG indices per tree (None or an
+    empty list tallies the root). Config keywords: lds_bytes,
+    max_tile_words, item_words, use_graph, key.
+    """
+
+    def __init__(self, pdags, shapes, tallies=None, **config):
+        L = load_library()
+        self._views = [_PdagView(p) for p in pdags]
+        n = len(pdags)
+        if _is_shape(shapes):
+            shapes = [shapes] * n
+        self._structs = (_Pdag * n)(*[v.struct for v in self._views])
+        self._shapes = (_Shape * n)(*[_shape(s) for s in shapes])
+        tallies = tallies or [[] for _ in range(n)]
+        off = np.zeros(n + 1, dtype=np.int32)
+        flat = []
+        for t, lst in enumerate(tallies):
+            flat.extend(int(x) for x in lst)
+            off[t + 1] = len(flat)
+        self._off = off
+        self._flat = np.ascontiguousarray(flat or [0], dtype=np.int32)
+        self._n_tally = [max(len(set(l)), 1) if l else 1 for l in tallies]
+        cfg = _BatchConfig()
+        L.mcs_batch_config_default(C.byref(cfg))
+        key = config.pop("key", None)
+        if key is not None:
+            cfg.key0, cfg.key1 = key[0] & 0xFFFFFFFF, key[1] & 0xFFFFFFFF
+        for k, v in config.items():
+            setattr(cfg, k, int(v))
+        self._cfg = cfg
+        handle = C.c_void_p()
+        _check(L.mcs_batch_create(self._structs, n, self._shapes, _p(self._off), _p(self._flat),
+                                  C.byref(cfg), C.byref(handle)))
+        self._h = handle
+        self.n_trees = n
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            load_library().mcs_batch_destroy(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def pass_(self, n: int = 1) -> None:
+        _check(load_library().mcs_batch_pass(self._h, n))
+
+    def wait(self) -> None:
+        _check(load_library().mcs_batch_wait(self._h))
+
+    def collect(self, tree: int) -> list[Tally]:
+        n = self._n_tally[tree]
+        out = (_Tally * n)()
+        _check(load_library().mcs_batch_collect(self._h, tree, out, n))
+        return [Tally._from(t) for t in out]
+
+    def info(self) -> dict:
+        i = _BatchInfo()
+        _check(load_library().mcs_batch_get_info(self._h, C.byref(i)))
+        return {f: getattr(i, f) for f, _ in _BatchInfo._fields_}
+
+    def tile_words(self, tree: int) -> int:
+        w = C.c_uint32()
+        _check(load_library().mcs_batch_tile_words(self._h, tree, C.byref(w)))
+        return w.value
+
+    def time_passes(self, n: int) -> float:
+        ms = C.c_double()
+        _check(load_library().mcs_batch_time_passes(self._h, n, C.byref(ms)))
+        return ms.value
