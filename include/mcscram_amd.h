@@ -243,17 +243,19 @@ mcs_status mcs_probability_analysis(const mcs_pdag *pdag, const mcs_settings *se
  * (src/mc/queue/layer_manager.tpp:344-375); each tree of a batch gives the
  * same tallies as its own engine (same Philox key, counters and sample
  * shape). A tree is evaluated whole in LDS: it must fit at one word per node
- * (about 20,000 nodes at the default budget); larger trees use
- * mcs_engine_create. Every tree advances by one iteration per pass.
+ * in 160 KiB (about 13,600 nodes); larger trees use mcs_engine_create. Every tree advances by one iteration per pass.
  */
 typedef struct mcs_batch mcs_batch;
 
 typedef struct mcs_batch_config {
-    uint32_t lds_bytes;       /* LDS per workgroup for a tile (0 = 64 KiB) */
+    uint32_t lds_bytes;       /* LDS per workgroup for a tile (0 = 64 KiB); raised
+                                 to fit the largest tree at one word, up to 160 KiB */
     uint32_t max_tile_words;  /* words per tile, power of two <= 64 (0 = 64) */
-    uint32_t item_words;      /* words per work item (0 = 4096) */
+    uint32_t item_words;      /* largest work item in words (0 = 4096); smaller
+                                 when the batch has too few tiles to fill the GPU */
     uint32_t use_graph;       /* 1: replay each pass as a captured hipGraph */
     uint32_t key0, key1;      /* Philox key (the reference key by default) */
+    uint32_t workgroup_lanes; /* 64..1024, a multiple of 64 (0 = 1024) */
 } mcs_batch_config;
 
 typedef struct mcs_batch_info {
@@ -261,6 +263,7 @@ typedef struct mcs_batch_info {
     uint32_t num_items;       /* work items per pass */
     uint32_t grid;            /* persistent workgroups per launch */
     uint32_t lds_bytes;       /* LDS per workgroup */
+    uint32_t workgroup_lanes;
     uint32_t iteration;       /* passes run */
     uint64_t device_bytes;
 } mcs_batch_info;

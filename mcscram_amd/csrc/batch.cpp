@@ -60,6 +60,7 @@ struct mcs_batch {
     uint32_t n_items = 0;
     uint32_t grid = 0;
     uint32_t lds_bytes = 0;
+    uint32_t lanes = 0;
     uint64_t n_ones = 0;
     uint64_t device_bytes = 0;
     mcs::FusedLaunch launch{};
@@ -85,7 +86,7 @@ mcs_status enqueue_pass(mcs_batch *b) {
         MCS_HIP(hipGraphLaunch(b->graph_exec, b->stream));
     } else {
         MCS_HIP(mcs::launch_fused_begin_pass(b->stream, b->launch.state));
-        MCS_HIP(mcs::launch_fused(b->stream, b->launch, b->grid, b->lds_bytes));
+        MCS_HIP(mcs::launch_fused(b->stream, b->launch, b->grid, b->lanes, b->lds_bytes));
     }
     ++b->iteration;
     return MCS_OK;
@@ -104,10 +105,14 @@ mcs_status build_batch(const mcs_pdag *pdags, int32_t n_trees, const mcs_sample_
     if (cfg.lds_bytes == 0) cfg.lds_bytes = 64u << 10;
     if (cfg.max_tile_words == 0) cfg.max_tile_words = 64;
     if (cfg.item_words == 0) cfg.item_words = 4096;
+    if (cfg.workgroup_lanes == 0) cfg.workgroup_lanes = 1024;
     if (cfg.max_tile_words > 64 || (cfg.max_tile_words & (cfg.max_tile_words - 1)))
         return fail(MCS_ERR_INVALID_ARGUMENT, "max_tile_words must be a power of two <= 64");
-    if (cfg.lds_bytes > (160u << 10))
+    if (cfg.lds_bytes > mcs::kMaxFusedLds)
         return fail(MCS_ERR_INVALID_ARGUMENT, "lds_bytes exceeds the 160 KiB of a CU");
+    if (cfg.workgroup_lanes > 1024 || cfg.workgroup_lanes % 64)
+        return fail(MCS_ERR_INVALID_ARGUMENT, "workgroup_lanes must be a multiple of 64 up to 1024");
+    b->lanes = cfg.workgroup_lanes;
 
     std::vector<mcs::FusedTree> ft;
     std::vector<mcs::EventDesc> ev;
@@ -118,6 +123,22 @@ mcs_status build_batch(const mcs_pdag *pdags, int32_t n_trees, const mcs_sample_
     };
     std::vector<Cost> cost;
     b->trees.resize(size_t(n_trees));
+    std::vector<mcs::Schedule> schedules(static_cast<size_t>(n_trees));
+    // Plans first: the launch's LDS budget must fit the largest tree.
+    uint64_t budget = cfg.lds_bytes;
+    for (int32_t t = 0; t < n_trees; ++t) {
+        TreeHost &th = b->trees[size_t(t)];
+        std::string err;
+        if (!mcs::build_plan(pdags[t], th.plan, err))
+            return fail(MCS_ERR_GRAPH, "tree " + std::to_string(t) + ": " + err);
+        schedules[size_t(t)] = mcs::build_schedule(th.plan, kBatchMaxFanIn);
+        const uint32_t n_nodes = uint32_t(th.plan.n_nodes() + schedules[size_t(t)].n_aux);
+        budget = std::max(budget, mcs::fused_lds_bytes(n_nodes, 1));
+        if (budget > mcs::kMaxFusedLds)
+            return fail(MCS_ERR_INVALID_ARGUMENT,
+                        "tree " + std::to_string(t) + " has " + std::to_string(n_nodes) +
+                            " nodes: too large for one LDS tile; use mcs_engine_create");
+    }
     for (int32_t t = 0; t < n_trees; ++t) {
         TreeHost &th = b->trees[size_t(t)];
         const mcs_sample_shape &sh = shapes[t];
@@ -126,11 +147,8 @@ mcs_status build_batch(const mcs_pdag *pdags, int32_t n_trees, const mcs_sample_
             th.n_bp / sh.batch_size != sh.bitpacks_per_batch || th.n_bp > (uint64_t(1) << 32))
             return fail(MCS_ERR_INVALID_ARGUMENT,
                         "tree " + std::to_string(t) + ": sample shape must be positive and at most 2^32 words");
-        std::string err;
-        if (!mcs::build_plan(pdags[t], th.plan, err))
-            return fail(MCS_ERR_GRAPH, "tree " + std::to_string(t) + ": " + err);
         const mcs::Plan &plan = th.plan;
-        const mcs::Schedule sch = mcs::build_schedule(plan, kBatchMaxFanIn);
+        const mcs::Schedule &sch = schedules[size_t(t)];
         const int32_t n_ev = plan.n_events();
         const int32_t n_nodes = plan.n_nodes() + sch.n_aux;
 
@@ -157,11 +175,7 @@ mcs_status build_batch(const mcs_pdag *pdags, int32_t n_trees, const mcs_sample_
 
         // Tile width: the widest power of two whose tile fits the budget.
         uint32_t w = cfg.max_tile_words;
-        while (w > 1 && mcs::fused_lds_bytes(uint32_t(n_nodes), w) > cfg.lds_bytes) w >>= 1;
-        if (mcs::fused_lds_bytes(uint32_t(n_nodes), w) > cfg.lds_bytes)
-            return fail(MCS_ERR_INVALID_ARGUMENT,
-                        "tree " + std::to_string(t) + " has " + std::to_string(n_nodes) +
-                            " nodes: too large for one LDS tile; use mcs_engine_create");
+        while (w > 1 && mcs::fused_lds_bytes(uint32_t(n_nodes), w) > budget) w >>= 1;
         th.tile_words = w;
         b->lds_bytes = std::max<uint32_t>(b->lds_bytes, uint32_t(mcs::fused_lds_bytes(uint32_t(n_nodes), w)));
 
@@ -217,13 +231,30 @@ mcs_status build_batch(const mcs_pdag *pdags, int32_t n_trees, const mcs_sample_
             return fail(MCS_ERR_INVALID_ARGUMENT, "batch too large");
     }
 
-    // Work items, largest first.
+    // Device and occupancy: the grid is every workgroup slot the LDS and
+    // register use allow.
+    MCS_HIP(hipGetDevice(&b->device));
+    MCS_HIP(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
+    int per_cu = 0;
+    MCS_HIP(mcs::fused_max_workgroups_per_cu(b->lanes, b->lds_bytes, &per_cu));
+    if (per_cu <= 0)
+        return fail(MCS_ERR_INVALID_ARGUMENT, "LDS and workgroup size leave no workgroup per CU");
+    hipDeviceProp_t prop;
+    MCS_HIP(hipGetDeviceProperties(&prop, b->device));
+    const uint64_t slots = uint64_t(per_cu) * uint64_t(prop.multiProcessorCount);
+
+    // Work items of whole tiles, at most item_words each, and small enough
+    // that there are four per workgroup slot; largest first.
+    uint64_t total_words = 0;
+    for (const TreeHost &th : b->trees) total_words += th.n_bp;
+    const uint64_t item_words = std::max<uint64_t>(
+        1, std::min<uint64_t>(cfg.item_words, total_words / std::max<uint64_t>(4 * slots, 1)));
     std::vector<mcs::WorkItem> items;
     std::vector<double> item_cost;
     for (int32_t t = 0; t < n_trees; ++t) {
         const TreeHost &th = b->trees[size_t(t)];
         const uint64_t n_tiles = (th.n_bp + th.tile_words - 1) / th.tile_words;
-        const uint64_t per_item = std::max<uint64_t>(1, cfg.item_words / th.tile_words);
+        const uint64_t per_item = std::max<uint64_t>(1, item_words / th.tile_words);
         for (uint64_t first = 0; first < n_tiles; first += per_item) {
             const uint64_t n = std::min(per_item, n_tiles - first);
             items.push_back({uint32_t(t), uint32_t(first), uint32_t(n), 0});
@@ -239,17 +270,7 @@ mcs_status build_batch(const mcs_pdag *pdags, int32_t n_trees, const mcs_sample_
     sorted.reserve(items.size());
     for (size_t i : order) sorted.push_back(items[i]);
     b->n_items = uint32_t(sorted.size());
-
-    // Device.
-    MCS_HIP(hipGetDevice(&b->device));
-    MCS_HIP(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
-    int per_cu = 0;
-    MCS_HIP(mcs::fused_max_workgroups_per_cu(b->lds_bytes, &per_cu));
-    if (per_cu <= 0) return fail(MCS_ERR_INVALID_ARGUMENT, "LDS budget leaves no workgroup per CU");
-    hipDeviceProp_t prop;
-    MCS_HIP(hipGetDeviceProperties(&prop, b->device));
-    b->grid = uint32_t(std::min<uint64_t>(uint64_t(per_cu) * uint64_t(prop.multiProcessorCount),
-                                          b->n_items));
+    b->grid = uint32_t(std::min<uint64_t>(slots, b->n_items));
 
     mcs::FusedTree *d_trees = nullptr;
     mcs::EventDesc *d_ev = nullptr;
@@ -297,7 +318,7 @@ mcs_status build_batch(const mcs_pdag *pdags, int32_t n_trees, const mcs_sample_
     if (cfg.use_graph) {
         MCS_HIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
         hipError_t e1 = mcs::launch_fused_begin_pass(b->stream, L.state);
-        hipError_t e2 = mcs::launch_fused(b->stream, L, b->grid, b->lds_bytes);
+        hipError_t e2 = mcs::launch_fused(b->stream, L, b->grid, b->lanes, b->lds_bytes);
         hipError_t e3 = hipStreamEndCapture(b->stream, &b->graph);
         MCS_HIP(e1);
         MCS_HIP(e2);
@@ -320,6 +341,7 @@ void mcs_batch_config_default(mcs_batch_config *cfg) {
     cfg->use_graph = 1;
     cfg->key0 = MCS_REFERENCE_KEY0;
     cfg->key1 = MCS_REFERENCE_KEY1;
+    cfg->workgroup_lanes = 1024;
 }
 
 mcs_status mcs_batch_create(const mcs_pdag *pdags, int32_t n_trees, const mcs_sample_shape *shapes,
@@ -379,6 +401,7 @@ mcs_status mcs_batch_get_info(mcs_batch *batch, mcs_batch_info *out) {
     out->num_items = batch->n_items;
     out->grid = batch->grid;
     out->lds_bytes = batch->lds_bytes;
+    out->workgroup_lanes = batch->lanes;
     out->iteration = batch->iteration;
     out->device_bytes = batch->device_bytes;
     return MCS_OK;

@@ -67,11 +67,15 @@ inline uint64_t fused_lds_bytes(uint32_t n_nodes, uint32_t w) {
     return uint64_t(n_nodes) * w * 8u + uint64_t(n_nodes) * 4u;
 }
 
-hipError_t fused_max_workgroups_per_cu(uint32_t lds_bytes, int *out);
+hipError_t fused_max_workgroups_per_cu(uint32_t lanes, uint32_t lds_bytes, int *out);
 // Starts one pass: queue head = 0, iteration += 1 (a kernel, so the pair can
 // be captured in a graph).
 hipError_t launch_fused_begin_pass(hipStream_t stream, uint32_t *state);
-hipError_t launch_fused(hipStream_t stream, const FusedLaunch &l, uint32_t grid,
+hipError_t launch_fused(hipStream_t stream, const FusedLaunch &l, uint32_t grid, uint32_t lanes,
                         uint32_t lds_bytes);
+
+// Largest dynamic LDS a workgroup may ask for on a CDNA4 CU (160 KiB, less
+// the kernel's static LDS word).
+constexpr uint32_t kMaxFusedLds = (160u << 10) - 64u;
 
 }  // namespace mcs

@@ -1,6 +1,6 @@
 // Persistent whole-tree-in-LDS kernel for batches of small trees (CDNA4).
 //
-// Workgroups of 256 lanes pull work items (tree, tiles) from a queue in HBM.
+// Workgroups of 64 to 1024 lanes pull work items (tree, tiles) from a queue in HBM.
 // For each tile of W words a workgroup samples every basic event of the tree
 // into LDS, then evaluates the gate layers from LDS with a workgroup barrier
 // between layers. Lane l of a step owns word (l mod W) of node (l / W), so
@@ -20,7 +20,7 @@
 namespace mcs {
 namespace {
 
-constexpr uint32_t kFusedLanes = 256;
+constexpr uint32_t kFusedMaxLanes = 1024;
 
 // Sum of v over the aligned group of w lanes (w a power of two <= 64).
 __device__ __forceinline__ uint32_t group_sum(uint32_t v, uint32_t w) {
@@ -88,10 +88,11 @@ __global__ void fused_begin_pass(uint32_t *state) {
     }
 }
 
-__global__ __launch_bounds__(256) void fused_trees(FusedLaunch L) {
+__global__ __launch_bounds__(1024) void fused_trees(FusedLaunch L) {
     extern __shared__ uint64_t lds[];
     __shared__ uint32_t s_item;
     const uint32_t tid = threadIdx.x;
+    const uint32_t lanes = blockDim.x;  // a multiple of 64
     const uint32_t iteration = L.state[1];
     const uint32_t c3base = (iteration + 1u) << 6;
     for (;;) {
@@ -105,7 +106,7 @@ __global__ __launch_bounds__(256) void fused_trees(FusedLaunch L) {
         const uint32_t log2_w = T.log2_w;
         const uint32_t W = 1u << log2_w;
         uint32_t *cnt = reinterpret_cast<uint32_t *>(lds + (uint64_t(T.n_nodes) << log2_w));
-        for (uint32_t n = tid; n < T.n_nodes; n += kFusedLanes) cnt[n] = 0;
+        for (uint32_t n = tid; n < T.n_nodes; n += lanes) cnt[n] = 0;
         __syncthreads();
         const EventDesc *events = L.events + T.ev_begin;
         const FusedGate *gates = L.gates + T.gate_begin;
@@ -114,7 +115,7 @@ __global__ __launch_bounds__(256) void fused_trees(FusedLaunch L) {
             const uint64_t base = uint64_t(wi.first_tile + tile) << log2_w;
             // Basic events of the tile.
             const uint32_t n_ev_items = T.n_ev << log2_w;
-            for (uint32_t idx = tid; idx < n_ev_items; idx += kFusedLanes) {
+            for (uint32_t idx = tid; idx < n_ev_items; idx += lanes) {
                 const uint32_t e = idx >> log2_w, w = idx & (W - 1u);
                 const uint64_t s = base + w;
                 const EventDesc d = events[e];
@@ -138,7 +139,7 @@ __global__ __launch_bounds__(256) void fused_trees(FusedLaunch L) {
             for (uint32_t l = 0; l < T.n_layers; ++l) {
                 const uint32_t g0 = layer_off[l];
                 const uint32_t n_items = (layer_off[l + 1] - g0) << log2_w;
-                for (uint32_t idx = tid; idx < n_items; idx += kFusedLanes) {
+                for (uint32_t idx = tid; idx < n_items; idx += lanes) {
                     const uint32_t g = g0 + (idx >> log2_w), w = idx & (W - 1u);
                     const FusedGate gd = gates[g];
                     const uint64_t r = eval_gate_word(gd, L.args + gd.arg_begin, lds, log2_w, w);
@@ -153,7 +154,7 @@ __global__ __launch_bounds__(256) void fused_trees(FusedLaunch L) {
                 __syncthreads();
             }
         }
-        for (uint32_t n = tid; n < T.n_nodes; n += kFusedLanes) {
+        for (uint32_t n = tid; n < T.n_nodes; n += lanes) {
             const uint32_t c = cnt[n];
             if (c) atomicAdd(L.ones + T.ones_begin + n, (unsigned long long)c);
         }
@@ -161,8 +162,8 @@ __global__ __launch_bounds__(256) void fused_trees(FusedLaunch L) {
     }
 }
 
-hipError_t fused_max_workgroups_per_cu(uint32_t lds_bytes, int *out) {
-    return hipOccupancyMaxActiveBlocksPerMultiprocessor(out, fused_trees, int(kFusedLanes),
+hipError_t fused_max_workgroups_per_cu(uint32_t lanes, uint32_t lds_bytes, int *out) {
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(out, fused_trees, int(lanes),
                                                         size_t(lds_bytes));
 }
 
@@ -171,10 +172,11 @@ hipError_t launch_fused_begin_pass(hipStream_t stream, uint32_t *state) {
     return hipGetLastError();
 }
 
-hipError_t launch_fused(hipStream_t stream, const FusedLaunch &l, uint32_t grid,
+hipError_t launch_fused(hipStream_t stream, const FusedLaunch &l, uint32_t grid, uint32_t lanes,
                         uint32_t lds_bytes) {
     if (grid == 0) return hipSuccess;
-    hipLaunchKernelGGL(fused_trees, dim3(grid), dim3(kFusedLanes), lds_bytes, stream, l);
+    if (lanes == 0 || lanes > kFusedMaxLanes || lanes % 64u) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(fused_trees, dim3(grid), dim3(lanes), lds_bytes, stream, l);
     return hipGetLastError();
 }
 
