@@ -58,7 +58,7 @@ def test_batch_matches_oracle_per_tree(gpu):
 @pytest.mark.gpu
 @pytest.mark.parametrize("cfg", [dict(use_graph=0), dict(lds_bytes=8192, item_words=64),
                                  dict(max_tile_words=1), dict(max_tile_words=16, item_words=100000),
-                                 dict(workgroup_lanes=64), dict(workgroup_lanes=256, lds_bytes=160 << 10)])
+                                 dict(workgroup_lanes=64), dict(workgroup_lanes=256, lds_bytes=150 << 10)])
 def test_batch_configs_give_identical_tallies(gpu, cfg):
     trees = _mixed_trees()[:5]
     widths = _check_batch(trees, [3000] * 5, n_pass=2, **cfg)
