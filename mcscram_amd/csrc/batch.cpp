@@ -151,6 +151,10 @@ mcs_status build_batch(const mcs_pdag *pdags, int32_t n_trees, const mcs_sample_
         const mcs::Schedule &sch = schedules[size_t(t)];
         const int32_t n_ev = plan.n_events();
         const int32_t n_nodes = plan.n_nodes() + sch.n_aux;
+        if (sch.layers.size() > mcs::kMaxFusedLayers)
+            return fail(MCS_ERR_INVALID_ARGUMENT,
+                        "tree " + std::to_string(t) + " is deeper than " +
+                            std::to_string(mcs::kMaxFusedLayers) + " layers; use mcs_engine_create");
 
         // Tallies.
         std::unordered_map<int32_t, int32_t> node_of_index;
@@ -210,14 +214,20 @@ mcs_status build_batch(const mcs_pdag *pdags, int32_t n_trees, const mcs_sample_
             layer_off.push_back(local);
             for (const mcs::ExecGate &g : layer) {
                 mcs::FusedGate fg{};
-                fg.arg_begin = uint32_t(args.size());
                 const uint32_t n_in = uint32_t(g.pos_in.size() + g.neg_in.size());
                 fg.counts = n_in | uint32_t(g.pos_in.size()) << 16;
                 const bool tal = g.out < plan.n_nodes() && tallied[size_t(g.out)];
                 fg.type_k_flags = uint32_t(g.type) | (uint32_t(uint8_t(g.k)) << 8) |
                                   (tal ? mcs::kGateTallied : 0u);
+                std::vector<uint32_t> slots;
                 for (const auto *v : {&g.pos_in, &g.neg_in})
-                    for (int32_t i : *v) args.push_back(uint32_t(th.slot_of_node[size_t(i)]));
+                    for (int32_t i : *v) slots.push_back(uint32_t(th.slot_of_node[size_t(i)]));
+                if (n_in <= mcs::kInlineInputs) {
+                    for (uint32_t j = 0; j < n_in; ++j) fg.in[j] = slots[j];
+                } else {
+                    fg.in[0] = uint32_t(args.size());
+                    args.insert(args.end(), slots.begin(), slots.end());
+                }
                 gates.push_back(fg);
                 fan += n_in;
                 ++local;

@@ -29,11 +29,14 @@ struct FusedTree {
     uint32_t ones_begin;   // first one-bit counter (one per node)
 };
 
-struct FusedGate {         // 16 bytes, evaluation order
-    uint32_t arg_begin;    // into args (node ids of the same tree)
+constexpr uint32_t kInlineInputs = 6;
+
+struct FusedGate {         // 32 bytes, evaluation order
     uint32_t counts;       // n_in | n_pos << 16
     uint32_t type_k_flags; // connective | k8 << 8 | kGateTallied
-    uint32_t reserved;
+    // LDS node slots of the inputs when n_in <= kInlineInputs; otherwise
+    // in[0] is the offset of the inputs in the args array.
+    uint32_t in[kInlineInputs];
 };
 
 struct WorkItem {
@@ -74,8 +77,10 @@ hipError_t launch_fused_begin_pass(hipStream_t stream, uint32_t *state);
 hipError_t launch_fused(hipStream_t stream, const FusedLaunch &l, uint32_t grid, uint32_t lanes,
                         uint32_t lds_bytes);
 
-// Largest dynamic LDS a workgroup may ask for on a CDNA4 CU (160 KiB, less
-// the kernel's static LDS word).
-constexpr uint32_t kMaxFusedLds = (160u << 10) - 64u;
+// Deepest tree (gate layers of the schedule) the kernel stages in LDS.
+constexpr uint32_t kMaxFusedLayers = 1023;
+// Largest dynamic LDS a workgroup may ask for on a CDNA4 CU: 160 KiB less
+// the kernel's static LDS (queue slot and layer offsets).
+constexpr uint32_t kMaxFusedLds = (160u << 10) - 4u * (kMaxFusedLayers + 1u) - 64u;
 
 }  // namespace mcs

@@ -21,6 +21,7 @@ namespace mcs {
 namespace {
 
 constexpr uint32_t kFusedMaxLanes = 1024;
+static_assert(sizeof(FusedGate) == 32, "gate descriptors are loaded as two 16-byte words");
 
 // Sum of v over the aligned group of w lanes (w a power of two <= 64).
 __device__ __forceinline__ uint32_t group_sum(uint32_t v, uint32_t w) {
@@ -33,50 +34,92 @@ __device__ __forceinline__ uint64_t lds_word(const uint64_t *lds, uint32_t node,
     return lds[(node << log2_w) + w];
 }
 
-__device__ __forceinline__ uint64_t eval_gate_word(const FusedGate &g,
-                                                   const uint32_t *__restrict__ a,
-                                                   const uint64_t *lds, uint32_t log2_w,
-                                                   uint32_t w) {
-    const uint32_t n_in = g.counts & 0xFFFFu;
-    const uint32_t n_pos = g.counts >> 16;
-    const uint32_t type = g.type_k_flags & 0xFFu;
-    const uint32_t k8 = (g.type_k_flags >> 8) & 0xFFu;
+// Gate value from its inputs (negated inputs complemented): `get(i, acc)`
+// hands input i to `acc`, in order, for i < n_in.
+template <typename Each>
+__device__ __forceinline__ uint64_t eval_inputs(Each each, uint32_t type, uint32_t k8) {
     uint64_t r;
     if (type == 2u) {  // ATLEAST: carry-save planes of the per-trial count
         uint64_t c[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-        for (uint32_t i = 0; i < n_in; ++i) {
-            uint64_t v = lds_word(lds, a[i], log2_w, w);
-            if (i >= n_pos) v = ~v;
+        each([&](uint64_t v) {
 #pragma unroll
             for (int b = 0; b < 8; ++b) {
                 const uint64_t t = c[b] & v;
                 c[b] ^= v;
                 v = t;
             }
-        }
+        });
         r = atleast_compare<uint64_t, 8>(c, k8);
     } else if (type == 4u || type == 7u) {  // NOT, NULL: the last input
         r = 0;
-        if (n_in) {
-            r = lds_word(lds, a[n_in - 1], log2_w, w);
-            if (n_in - 1 >= n_pos) r = ~r;
-        }
+        each([&](uint64_t v) { r = v; });
+    } else if (type == 0u || type == 5u) {
+        r = ~0ull;
+        each([&](uint64_t v) { r &= v; });
+    } else if (type == 3u) {
+        r = 0;
+        each([&](uint64_t v) { r ^= v; });
     } else {
-        const bool and_like = (type == 0u || type == 5u);
-        r = and_like ? ~0ull : 0ull;
-        for (uint32_t i = 0; i < n_in; ++i) {
-            uint64_t v = lds_word(lds, a[i], log2_w, w);
-            if (i >= n_pos) v = ~v;
-            if (and_like)
-                r &= v;
-            else if (type == 3u)
-                r ^= v;
-            else
-                r |= v;
-        }
+        r = 0;
+        each([&](uint64_t v) { r |= v; });
     }
     if (type == 4u || type == 5u || type == 6u) r = ~r;  // NOT, NAND, NOR
     return r;
+}
+
+// A gate descriptor held in registers: two 16-byte words,
+// a = {counts, type_k_flags, in0, in1}, b = {in2, in3, in4, in5}.
+struct GateRegs {
+    uint4 a, b;
+};
+
+__device__ __forceinline__ GateRegs load_gate(const FusedGate *g) {
+    const uint4 *p = reinterpret_cast<const uint4 *>(g);
+    return {p[0], p[1]};
+}
+
+__device__ __forceinline__ uint64_t eval_gate_word(const GateRegs &g,
+                                                   const uint32_t *__restrict__ args,
+                                                   const uint64_t *lds, uint32_t log2_w,
+                                                   uint32_t w) {
+    const uint32_t n_in = g.a.x & 0xFFFFu;
+    const uint32_t n_pos = g.a.x >> 16;
+    const uint32_t type = g.a.y & 0xFFu;
+    const uint32_t k8 = (g.a.y >> 8) & 0xFFu;
+    if (n_in <= kInlineInputs) {
+        // Inputs in the descriptor: the LDS reads of all inputs are issued
+        // together.
+        const uint32_t slot[kInlineInputs] = {g.a.z, g.a.w, g.b.x, g.b.y, g.b.z, g.b.w};
+        uint64_t v[kInlineInputs];
+#pragma unroll
+        for (uint32_t j = 0; j < kInlineInputs; ++j) {
+            const uint64_t x = j < n_in ? lds_word(lds, slot[j], log2_w, w) : 0ull;
+            v[j] = j >= n_pos ? ~x : x;
+        }
+        return eval_inputs(
+            [&](auto &&acc) {
+#pragma unroll
+                for (uint32_t j = 0; j < kInlineInputs; ++j)
+                    if (j < n_in) acc(v[j]);
+            },
+            type, k8);
+    }
+    const uint32_t *a = args + g.a.z;
+    return eval_inputs(
+        [&](auto &&acc) {
+            for (uint32_t i = 0; i < n_in; ++i) {
+                const uint64_t x = lds_word(lds, a[i], log2_w, w);
+                acc(i >= n_pos ? ~x : x);
+            }
+        },
+        type, k8);
+}
+
+// Workgroup barrier for LDS traffic only: waits for this wave's LDS
+// operations, then for every wave, and leaves global loads in flight (the
+// kernel stores nothing to global memory inside a tile).
+__device__ __forceinline__ void lds_barrier() {
+    __asm__ volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
 }
 
 }  // namespace
@@ -91,6 +134,7 @@ __global__ void fused_begin_pass(uint32_t *state) {
 __global__ __launch_bounds__(1024) void fused_trees(FusedLaunch L) {
     extern __shared__ uint64_t lds[];
     __shared__ uint32_t s_item;
+    __shared__ uint32_t s_layer_off[kMaxFusedLayers + 1];
     const uint32_t tid = threadIdx.x;
     const uint32_t lanes = blockDim.x;  // a multiple of 64
     const uint32_t iteration = L.state[1];
@@ -98,7 +142,8 @@ __global__ __launch_bounds__(1024) void fused_trees(FusedLaunch L) {
     for (;;) {
         if (tid == 0) s_item = atomicAdd(L.state, 1u);
         __syncthreads();
-        const uint32_t item = s_item;
+        // Wave-uniform: everything derived from it is loaded by the scalar unit.
+        const uint32_t item = __builtin_amdgcn_readfirstlane(s_item);
         __syncthreads();
         if (item >= L.n_items) break;
         const WorkItem wi = L.items[item];
@@ -110,48 +155,69 @@ __global__ __launch_bounds__(1024) void fused_trees(FusedLaunch L) {
         __syncthreads();
         const EventDesc *events = L.events + T.ev_begin;
         const FusedGate *gates = L.gates + T.gate_begin;
-        const uint32_t *layer_off = L.layer_off + T.layer_begin;
+        // Layer offsets of the tree, staged once per item.
+        for (uint32_t l = tid; l <= T.n_layers; l += lanes) s_layer_off[l] = L.layer_off[T.layer_begin + l];
+        __syncthreads();
+        const uint32_t *layer_off = s_layer_off;
         for (uint32_t tile = 0; tile < wi.n_tiles; ++tile) {
             const uint64_t base = uint64_t(wi.first_tile + tile) << log2_w;
-            // Basic events of the tile.
+            // Basic events of the tile; the next item's descriptor is loaded
+            // while the current word is computed. Descriptor fields:
+            // {ctr0, ctr1, threshold, mode}.
             const uint32_t n_ev_items = T.n_ev << log2_w;
+            const uint4 *ev4 = reinterpret_cast<const uint4 *>(events);
+            // Prefetch indices are clamped to the last event: every load is
+            // in bounds and unconditional.
+            const uint32_t last_ev = T.n_ev ? T.n_ev - 1u : 0u;
+            uint4 d = ev4[min(tid >> log2_w, last_ev)];
             for (uint32_t idx = tid; idx < n_ev_items; idx += lanes) {
                 const uint32_t e = idx >> log2_w, w = idx & (W - 1u);
+                const uint4 d_next = ev4[min((idx + lanes) >> log2_w, last_ev)];
                 const uint64_t s = base + w;
-                const EventDesc d = events[e];
-                const uint32_t mode = d.mode & 3u;
+                const uint32_t mode = d.w & 3u;
                 uint64_t word = 0;
                 if (s < T.n_bp) {
                     if (mode == kEventDraw)
-                        word = bernoulli_word(d.ctr0, d.ctr1, uint32_t(s) + 1u, c3base, L.key0,
-                                              L.key1, d.threshold);
+                        word = bernoulli_word(d.x, d.y, uint32_t(s) + 1u, c3base, L.key0, L.key1,
+                                              d.z);
                     else if (mode == kEventAlways)
                         word = ~0ull;
                 }
                 lds[idx] = word;
-                if (d.mode & kEventTallied) {
+                if (d.w & kEventTallied) {
                     const uint32_t pc = group_sum(uint32_t(__popcll(word)), W);
                     if (w == 0 && pc) atomicAdd(cnt + e, pc);
                 }
+                d = d_next;
             }
-            __syncthreads();
-            // Gate layers, each from LDS only.
+            // Gate layers, each from LDS only. The descriptor of a lane's
+            // first gate of layer l + 1 is loaded before layer l's barrier.
+            uint32_t g_lo = layer_off[0];
+            uint32_t g_hi = T.n_layers ? layer_off[1] : g_lo;
+            // Prefetch indices are clamped to the tree's last gate.
+            const uint32_t last_gate = layer_off[T.n_layers] - 1u;
+            GateRegs gd_next = load_gate(gates + min(g_lo + (tid >> log2_w), last_gate));
+            lds_barrier();
             for (uint32_t l = 0; l < T.n_layers; ++l) {
-                const uint32_t g0 = layer_off[l];
-                const uint32_t n_items = (layer_off[l + 1] - g0) << log2_w;
+                const uint32_t g0 = g_lo;
+                const uint32_t n_items = (g_hi - g0) << log2_w;
+                GateRegs gd = gd_next;
+                g_lo = g_hi;
+                g_hi = l + 2 <= T.n_layers ? layer_off[l + 2] : g_hi;
+                gd_next = load_gate(gates + min(g_lo + (tid >> log2_w), last_gate));
                 for (uint32_t idx = tid; idx < n_items; idx += lanes) {
                     const uint32_t g = g0 + (idx >> log2_w), w = idx & (W - 1u);
-                    const FusedGate gd = gates[g];
-                    const uint64_t r = eval_gate_word(gd, L.args + gd.arg_begin, lds, log2_w, w);
+                    if (idx != tid) gd = load_gate(gates + g);
+                    const uint64_t r = eval_gate_word(gd, L.args, lds, log2_w, w);
                     const uint32_t node = T.n_ev + g;
                     lds[(node << log2_w) + w] = r;
-                    if (gd.type_k_flags & kGateTallied) {
+                    if (gd.a.y & kGateTallied) {
                         const uint32_t pc =
                             group_sum(base + w < T.n_bp ? uint32_t(__popcll(r)) : 0u, W);
                         if (w == 0 && pc) atomicAdd(cnt + node, pc);
                     }
                 }
-                __syncthreads();
+                lds_barrier();
             }
         }
         for (uint32_t n = tid; n < T.n_nodes; n += lanes) {
