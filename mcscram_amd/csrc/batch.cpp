@@ -214,16 +214,27 @@ mcs_status build_batch(const mcs_pdag *pdags, int32_t n_trees, const mcs_sample_
             layer_off.push_back(local);
             for (const mcs::ExecGate &g : layer) {
                 mcs::FusedGate fg{};
-                const uint32_t n_in = uint32_t(g.pos_in.size() + g.neg_in.size());
-                fg.counts = n_in | uint32_t(g.pos_in.size()) << 16;
-                const bool tal = g.out < plan.n_nodes() && tallied[size_t(g.out)];
-                fg.type_k_flags = uint32_t(g.type) | (uint32_t(uint8_t(g.k)) << 8) |
-                                  (tal ? mcs::kGateTallied : 0u);
+                // Slots in the reference's input order (positive, then
+                // negated); NOT and NULL read only their last input.
                 std::vector<uint32_t> slots;
-                for (const auto *v : {&g.pos_in, &g.neg_in})
-                    for (int32_t i : *v) slots.push_back(uint32_t(th.slot_of_node[size_t(i)]));
+                for (int32_t i : g.pos_in) slots.push_back(uint32_t(th.slot_of_node[size_t(i)]));
+                for (int32_t i : g.neg_in)
+                    slots.push_back(uint32_t(th.slot_of_node[size_t(i)]) | mcs::kSlotNegated);
+                uint32_t type = g.type;
+                if (type == MCS_NOT || type == MCS_NULL) {
+                    if (!slots.empty()) slots = {slots.back()};
+                    type = type == MCS_NOT ? MCS_NOR : MCS_OR;
+                }
+                const uint32_t n_in = uint32_t(slots.size());
+                fg.n_in = n_in;
+                const bool tal = g.out < plan.n_nodes() && tallied[size_t(g.out)];
+                fg.type_k_flags = type | (uint32_t(uint8_t(g.k)) << 8) |
+                                  (tal ? mcs::kGateTallied : 0u);
                 if (n_in <= mcs::kInlineInputs) {
-                    for (uint32_t j = 0; j < n_in; ++j) fg.in[j] = slots[j];
+                    const bool and_like = type == MCS_AND || type == MCS_NAND;
+                    const uint32_t pad = uint32_t(n_nodes) + (and_like ? 1u : 0u);
+                    for (uint32_t j = 0; j < mcs::kInlineInputs; ++j)
+                        fg.in[j] = j < n_in ? slots[j] : pad;
                 } else {
                     fg.in[0] = uint32_t(args.size());
                     args.insert(args.end(), slots.begin(), slots.end());

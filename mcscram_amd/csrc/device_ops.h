@@ -46,15 +46,20 @@ __device__ __forceinline__ void philox_round(uint32_t &x0, uint32_t &x1, uint32_
 // XOR alone. So M1*c2 (round 1), M0*x0 (round 2) and M1*x2 (round 3) are the
 // same for all 16 generations and are computed once per word, and the
 // generation-dependent half of round 2 depends only on per-event values
-// (c0, c3base, g), which are wave-uniform here and go to the scalar unit.
+// (c0, c3base, g), which go to the scalar unit when they are wave-uniform.
 //
-// Packing: the four draws of a generation hit only if their minimum is below
-// the threshold; when no lane of the wave hits (most generations at the
-// small probabilities of fault trees) the compares and shifts are skipped by
-// one wave-uniform branch.
-__device__ __forceinline__ uint64_t bernoulli_word(uint32_t c0, uint32_t c1, uint32_t c2,
-                                                   uint32_t c3base, uint32_t k0, uint32_t k1,
-                                                   uint32_t thr) {
+// Packing, two ways:
+//  * kDense = false: the four draws of a generation hit only if their
+//    minimum is below the threshold; when no lane of the wave hits (most
+//    generations at the small probabilities of fault trees) the compares
+//    and shifts are skipped by one wave-uniform branch.
+//  * kDense = true: no branch; generations run from 15 down to 0 and each
+//    draw's comparison is shifted into the word (compare + shift-add), for
+//    waves where hits are frequent and the branch would always be taken.
+template <bool kDense>
+__device__ __forceinline__ uint64_t bernoulli_word_t(uint32_t c0, uint32_t c1, uint32_t c2,
+                                                     uint32_t c3base, uint32_t k0, uint32_t k1,
+                                                     uint32_t thr) {
     // Round 1: (c0, c1, c2, c3) with key (k0, k1).
     uint32_t hA, lA, hB, lB;
     mul_wide(c2, kPhiloxM1, hA, lA);  // per word
@@ -74,9 +79,8 @@ __device__ __forceinline__ uint64_t bernoulli_word(uint32_t c0, uint32_t c1, uin
     uint32_t hD, lD;
     mul_wide(b2, kPhiloxM1, hD, lD);
 
-    uint32_t lo_bits = 0, hi_bits = 0;
-#pragma unroll
-    for (uint32_t g = 0; g < 16; ++g) {
+    // Outputs (x0..x3) of generation g.
+    auto draw = [&](uint32_t g, uint32_t &x0, uint32_t &x1, uint32_t &x2, uint32_t &x3) {
         const uint32_t a2 = a2base ^ (c3base + g);  // per event
         uint32_t hE, lE;
         mul_wide(a2, kPhiloxM1, hE, lE);  // per event
@@ -84,24 +88,57 @@ __device__ __forceinline__ uint64_t bernoulli_word(uint32_t c0, uint32_t c1, uin
         const uint32_t b1 = lE;
         uint32_t hF, lF;
         mul_wide(b0, kPhiloxM0, hF, lF);
-        uint32_t x0 = hD ^ (b1 ^ k0r2);
-        uint32_t x1 = lD;
-        uint32_t x2 = xor3(hF, b3, k1r2);
-        uint32_t x3 = lF;
+        x0 = hD ^ (b1 ^ k0r2);
+        x1 = lD;
+        x2 = xor3(hF, b3, k1r2);
+        x3 = lF;
 #pragma unroll
         for (uint32_t r = 3; r < 10; ++r)
             philox_round(x0, x1, x2, x3, k0 + r * kPhiloxW0, k1 + r * kPhiloxW1);
-        const uint32_t m = min(min(x0, x1), min(x2, x3));
-        if (__builtin_amdgcn_ballot_w64(m < thr) != 0) {
-            const uint32_t nib = uint32_t(x0 < thr) | uint32_t(x1 < thr) << 1 |
-                                 uint32_t(x2 < thr) << 2 | uint32_t(x3 < thr) << 3;
-            if (g < 8)
-                lo_bits |= nib << (4u * g);
-            else
-                hi_bits |= nib << (4u * (g - 8u));
+    };
+
+    uint32_t lo_bits = 0, hi_bits = 0;
+    if (kDense) {
+#pragma unroll
+        for (int g = 15; g >= 0; --g) {
+            uint32_t x0, x1, x2, x3;
+            draw(uint32_t(g), x0, x1, x2, x3);
+            uint32_t &acc = g >= 8 ? hi_bits : lo_bits;
+            acc = acc * 2u + uint32_t(x3 < thr);
+            acc = acc * 2u + uint32_t(x2 < thr);
+            acc = acc * 2u + uint32_t(x1 < thr);
+            acc = acc * 2u + uint32_t(x0 < thr);
+        }
+    } else {
+#pragma unroll
+        for (uint32_t g = 0; g < 16; ++g) {
+            uint32_t x0, x1, x2, x3;
+            draw(g, x0, x1, x2, x3);
+            const uint32_t m = min(min(x0, x1), min(x2, x3));
+            if (__builtin_amdgcn_ballot_w64(m < thr) != 0) {
+                const uint32_t nib = uint32_t(x0 < thr) | uint32_t(x1 < thr) << 1 |
+                                     uint32_t(x2 < thr) << 2 | uint32_t(x3 < thr) << 3;
+                if (g < 8)
+                    lo_bits |= nib << (4u * g);
+                else
+                    hi_bits |= nib << (4u * (g - 8u));
+            }
         }
     }
     return uint64_t(hi_bits) << 32 | lo_bits;
+}
+
+// Thresholds at or above this (p >= 2^-10) make hits frequent enough that
+// the branch-free packing is cheaper for a wave holding such an event.
+constexpr uint32_t kDenseThreshold = 1u << 22;
+
+// The Bernoulli word with the packing chosen per wave.
+__device__ __forceinline__ uint64_t bernoulli_word(uint32_t c0, uint32_t c1, uint32_t c2,
+                                                   uint32_t c3base, uint32_t k0, uint32_t k1,
+                                                   uint32_t thr) {
+    if (__builtin_amdgcn_ballot_w64(thr >= kDenseThreshold) != 0)
+        return bernoulli_word_t<true>(c0, c1, c2, c3base, k0, k1, thr);
+    return bernoulli_word_t<false>(c0, c1, c2, c3base, k0, k1, thr);
 }
 
 __device__ __forceinline__ uint32_t wave_sum(uint32_t v) {

@@ -30,12 +30,17 @@ struct FusedTree {
 };
 
 constexpr uint32_t kInlineInputs = 6;
+constexpr uint32_t kSlotNegated = 1u << 31;
 
+// Input slots are LDS node slots | kSlotNegated for complemented inputs.
+// Slots n_nodes and n_nodes + 1 hold an all-zeros and an all-ones word.
 struct FusedGate {         // 32 bytes, evaluation order
-    uint32_t counts;       // n_in | n_pos << 16
-    uint32_t type_k_flags; // connective | k8 << 8 | kGateTallied
-    // LDS node slots of the inputs when n_in <= kInlineInputs; otherwise
-    // in[0] is the offset of the inputs in the args array.
+    uint32_t n_in;
+    uint32_t type_k_flags; // connective | k8 << 8 | kGateTallied; NOT and NULL
+                           // are sent as NOR and OR of their last input
+    // Input slots when n_in <= kInlineInputs, unused ones padded with the
+    // identity of the fold; otherwise in[0] is the offset of the slots in
+    // the args array.
     uint32_t in[kInlineInputs];
 };
 
@@ -65,9 +70,10 @@ struct FusedLaunch {
 
 namespace mcs {
 
-// LDS bytes a tree needs at tile width W.
+// LDS bytes a tree needs at tile width W: its nodes and the two identity
+// words, W words each, and one 32-bit counter per node.
 inline uint64_t fused_lds_bytes(uint32_t n_nodes, uint32_t w) {
-    return uint64_t(n_nodes) * w * 8u + uint64_t(n_nodes) * 4u;
+    return uint64_t(n_nodes + 2u) * w * 8u + uint64_t(n_nodes) * 4u;
 }
 
 hipError_t fused_max_workgroups_per_cu(uint32_t lanes, uint32_t lds_bytes, int *out);

@@ -34,41 +34,8 @@ __device__ __forceinline__ uint64_t lds_word(const uint64_t *lds, uint32_t node,
     return lds[(node << log2_w) + w];
 }
 
-// Gate value from its inputs (negated inputs complemented): `get(i, acc)`
-// hands input i to `acc`, in order, for i < n_in.
-template <typename Each>
-__device__ __forceinline__ uint64_t eval_inputs(Each each, uint32_t type, uint32_t k8) {
-    uint64_t r;
-    if (type == 2u) {  // ATLEAST: carry-save planes of the per-trial count
-        uint64_t c[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-        each([&](uint64_t v) {
-#pragma unroll
-            for (int b = 0; b < 8; ++b) {
-                const uint64_t t = c[b] & v;
-                c[b] ^= v;
-                v = t;
-            }
-        });
-        r = atleast_compare<uint64_t, 8>(c, k8);
-    } else if (type == 4u || type == 7u) {  // NOT, NULL: the last input
-        r = 0;
-        each([&](uint64_t v) { r = v; });
-    } else if (type == 0u || type == 5u) {
-        r = ~0ull;
-        each([&](uint64_t v) { r &= v; });
-    } else if (type == 3u) {
-        r = 0;
-        each([&](uint64_t v) { r ^= v; });
-    } else {
-        r = 0;
-        each([&](uint64_t v) { r |= v; });
-    }
-    if (type == 4u || type == 5u || type == 6u) r = ~r;  // NOT, NAND, NOR
-    return r;
-}
-
 // A gate descriptor held in registers: two 16-byte words,
-// a = {counts, type_k_flags, in0, in1}, b = {in2, in3, in4, in5}.
+// a = {n_in, type_k_flags, in0, in1}, b = {in2, in3, in4, in5}.
 struct GateRegs {
     uint4 a, b;
 };
@@ -78,41 +45,84 @@ __device__ __forceinline__ GateRegs load_gate(const FusedGate *g) {
     return {p[0], p[1]};
 }
 
+// The word of an input slot (node | negated << 31), complemented if negated.
+__device__ __forceinline__ uint64_t slot_word(const uint64_t *lds, uint32_t slot, uint32_t log2_w,
+                                              uint32_t w) {
+    const uint64_t x = lds[((slot & 0x7FFFFFFFu) << log2_w) + w];
+    const uint64_t neg = uint64_t(int64_t(int32_t(slot)) >> 31);  // all ones if negated
+    return x ^ neg;
+}
+
+// Carry-save count of the inputs, then count >= k8 on the P planes; k8 at
+// or above 2^P can never be met.
+template <int P, typename Each>
+__device__ __forceinline__ uint64_t atleast_planes(Each each, uint32_t k8) {
+    uint64_t c[P];
+#pragma unroll
+    for (int b = 0; b < P; ++b) c[b] = 0;
+    each([&](uint64_t v) {
+#pragma unroll
+        for (int b = 0; b < P; ++b) {
+            const uint64_t t = c[b] & v;
+            c[b] ^= v;
+            v = t;
+        }
+    });
+    if (P < 8 && k8 >= (1u << P)) return 0;
+    return atleast_compare<uint64_t, P>(c, k8);
+}
+
+// Gate word. Descriptors are normalised on the host: NOT and NULL arrive as
+// NOR and OR of their last input, and the unused inline slots point at an
+// all-zeros (OR, XOR, NOR, ATLEAST) or all-ones (AND, NAND) LDS word, so the
+// six inline inputs fold without conditions.
 __device__ __forceinline__ uint64_t eval_gate_word(const GateRegs &g,
                                                    const uint32_t *__restrict__ args,
                                                    const uint64_t *lds, uint32_t log2_w,
                                                    uint32_t w) {
-    const uint32_t n_in = g.a.x & 0xFFFFu;
-    const uint32_t n_pos = g.a.x >> 16;
+    const uint32_t n_in = g.a.x;
     const uint32_t type = g.a.y & 0xFFu;
     const uint32_t k8 = (g.a.y >> 8) & 0xFFu;
+    uint64_t r;
     if (n_in <= kInlineInputs) {
-        // Inputs in the descriptor: the LDS reads of all inputs are issued
-        // together.
         const uint32_t slot[kInlineInputs] = {g.a.z, g.a.w, g.b.x, g.b.y, g.b.z, g.b.w};
         uint64_t v[kInlineInputs];
 #pragma unroll
-        for (uint32_t j = 0; j < kInlineInputs; ++j) {
-            const uint64_t x = j < n_in ? lds_word(lds, slot[j], log2_w, w) : 0ull;
-            v[j] = j >= n_pos ? ~x : x;
-        }
-        return eval_inputs(
-            [&](auto &&acc) {
+        for (uint32_t j = 0; j < kInlineInputs; ++j) v[j] = slot_word(lds, slot[j], log2_w, w);
+        if (type == 2u) {  // at most six inputs: three planes
+            r = atleast_planes<3>(
+                [&](auto &&acc) {
 #pragma unroll
-                for (uint32_t j = 0; j < kInlineInputs; ++j)
-                    if (j < n_in) acc(v[j]);
-            },
-            type, k8);
+                    for (uint32_t j = 0; j < kInlineInputs; ++j) acc(v[j]);
+                },
+                k8);
+        } else if (type == 0u || type == 5u) {
+            r = v[0] & v[1] & v[2] & v[3] & v[4] & v[5];
+        } else if (type == 3u) {
+            r = v[0] ^ v[1] ^ v[2] ^ v[3] ^ v[4] ^ v[5];
+        } else {
+            r = v[0] | v[1] | v[2] | v[3] | v[4] | v[5];
+        }
+    } else {
+        const uint32_t *a = args + g.a.z;
+        auto each = [&](auto &&acc) {
+            for (uint32_t i = 0; i < n_in; ++i) acc(slot_word(lds, a[i], log2_w, w));
+        };
+        if (type == 2u) {
+            r = atleast_planes<8>(each, k8);  // the reference's 8-bit count
+        } else if (type == 0u || type == 5u) {
+            r = ~0ull;
+            each([&](uint64_t v) { r &= v; });
+        } else if (type == 3u) {
+            r = 0;
+            each([&](uint64_t v) { r ^= v; });
+        } else {
+            r = 0;
+            each([&](uint64_t v) { r |= v; });
+        }
     }
-    const uint32_t *a = args + g.a.z;
-    return eval_inputs(
-        [&](auto &&acc) {
-            for (uint32_t i = 0; i < n_in; ++i) {
-                const uint64_t x = lds_word(lds, a[i], log2_w, w);
-                acc(i >= n_pos ? ~x : x);
-            }
-        },
-        type, k8);
+    if (type == 5u || type == 6u) r = ~r;  // NAND, NOR (and NOT, sent as NOR)
+    return r;
 }
 
 // Workgroup barrier for LDS traffic only: waits for this wave's LDS
@@ -150,8 +160,12 @@ __global__ __launch_bounds__(1024) void fused_trees(FusedLaunch L) {
         const FusedTree T = L.trees[wi.tree];
         const uint32_t log2_w = T.log2_w;
         const uint32_t W = 1u << log2_w;
-        uint32_t *cnt = reinterpret_cast<uint32_t *>(lds + (uint64_t(T.n_nodes) << log2_w));
+        // LDS: W words per node, then the all-zeros and all-ones words that
+        // pad unused gate inputs, then one counter per node.
+        uint32_t *cnt = reinterpret_cast<uint32_t *>(lds + (uint64_t(T.n_nodes + 2u) << log2_w));
         for (uint32_t n = tid; n < T.n_nodes; n += lanes) cnt[n] = 0;
+        for (uint32_t i = tid; i < 2u * W; i += lanes)
+            lds[(uint64_t(T.n_nodes) << log2_w) + i] = i < W ? 0ull : ~0ull;
         __syncthreads();
         const EventDesc *events = L.events + T.ev_begin;
         const FusedGate *gates = L.gates + T.gate_begin;

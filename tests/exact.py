@@ -32,7 +32,14 @@ def exact_probabilities(pd: Pdag) -> dict[int, float]:
         if idx in value:
             return value[idx]
         s = slot_of[idx]
-        ins = [gate(abs(int(a))) if a > 0 else ~gate(abs(int(a))) for a in pd.gate_args(s)]
+        args = [int(a) for a in pd.gate_args(s)]
+        # The reference's input order: positive inputs, then negated ones,
+        # variables before gates within each (kernel_builder.h:298-426);
+        # NOT and NULL read the last of them (gate.h:321-351).
+        is_var = lambda a: abs(a) in value and abs(a) not in slot_of  # noqa: E731
+        order = [a for a in args if a > 0 and is_var(a)] + [a for a in args if a > 0 and not is_var(a)] + \
+                [a for a in args if a < 0 and is_var(a)] + [a for a in args if a < 0 and not is_var(a)]
+        ins = [gate(abs(a)) if a > 0 else ~gate(abs(a)) for a in order]
         t = int(pd.gate_type[s])
         if t in (AND, NAND):
             r = np.logical_and.reduce(ins)
@@ -43,7 +50,7 @@ def exact_probabilities(pd: Pdag) -> dict[int, float]:
         elif t == ATLEAST:
             r = np.sum(ins, axis=0) >= int(pd.gate_k[s])
         elif t in (NOT, NULL):
-            (r,) = ins
+            r = ins[-1] if ins else np.zeros_like(weight, dtype=bool)
         else:
             raise ValueError(f"connective {t}")
         if t in (NAND, NOR, NOT):

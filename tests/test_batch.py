@@ -85,3 +85,37 @@ def test_batch_root_only_default_and_info(gpu):
     for pd, t in zip(trees, roots):
         want, _ = orc.run(pd, 2048, n_iters=3)
         assert t.index == pd.root and t.num_one_bits == int(want[0])
+
+
+def _edge_tree():
+    """NOT/NULL with several inputs (the last one counts), a negated last
+    input, ATLEAST with k = 0, k > n and k = 7 of 6, XOR with negations, and
+    a gate wider than the inline inputs."""
+    from mcscram_amd.pdag import AND, ATLEAST, NAND, NOR, NOT, NULL, OR, XOR, Pdag
+    vs = [(2 + i, 0.05 + 0.1 * (i % 9)) for i in range(12)]
+    g = [
+        (20, NOT, 0, [2, 3, 4]), (21, NULL, 0, [5, -6]), (22, NOT, 0, [7, -8]),
+        (23, ATLEAST, 0, [2, 3, 4]), (24, ATLEAST, 5, [2, 3, 4, 5]),
+        (25, ATLEAST, 7, [2, 3, 4, 5, 6, 7]), (26, ATLEAST, 3, [2, -3, 4, -5, 6, 7]),
+        (27, XOR, 0, [-2, 3, -4]), (28, NAND, 0, [9, -10, 11]), (29, NOR, 0, [12, -13]),
+        (30, AND, 0, [-2, -3, -4, -5, -6, -7, -8]), (31, ATLEAST, 4, list(range(2, 14))),
+        (32, OR, 0, [20, 21, 22, 23, 24, 25, 26, 27, 28, 29, 30, 31]),
+    ]
+    return Pdag.from_lists(vs, g, 32)
+
+
+@pytest.mark.gpu
+def test_batch_gate_normalisation_edges(gpu):
+    _check_batch([_edge_tree(), _edge_tree()], [1000, (2, 512)], n_pass=2)
+
+
+def test_edge_tree_oracle_against_enumeration():
+    from tests.exact import exact_probabilities
+    pd = _edge_tree()
+    exact = exact_probabilities(pd)
+    tally = [int(g) for g in pd.gate_index]
+    ones, _ = orc.run(pd, 1 << 13, tally=tally)
+    for idx, o in zip(tally, ones):
+        m = int(o) / (1 << 19)
+        se = max(np.sqrt(m * (1 - m) / (1 << 19)), 1 / (1 << 19))
+        assert abs(m - exact[idx]) <= 3.5 * se, (idx, m, exact[idx])
