@@ -120,11 +120,14 @@ def bench_fused(args, mc) -> int:
         "config": {"workload": name, "trees": len(trees), "basic_events": n_be, "gates": n_g,
                    "words_per_pass": words, "trials_per_tree_per_pass": words * 64,
                    "batch_config": cfg, "grid": info["grid"], "lds_bytes": info["lds_bytes"],
-                   "work_items": info["num_items"]},
+                   "work_items": info["num_items"], "fused_trees": info["num_fused_trees"],
+                   "layered_trees": info["num_layered_trees"],
+                   "layered_launches_per_pass": info["layered_launches"]},
         "node_evals_per_s": (n_g + n_be) * trials / wall,
         "device_ms_per_pass": dev_ms / args.steps,
         "philox_words_per_s": n_be * words * args.steps / wall,
-        "hbm_bytes_per_pass": "tallies only (trees live in LDS)",
+        "hbm": "fused trees: tallies only (the tree lives in LDS); layered trees: "
+               "materialised rows, next tree sampled beside each tree's gates",
     }
     if not args.no_cpu:
         # CPU oracle on a fixed subset of the trees, bit-exact check beside it.

@@ -248,18 +248,22 @@ mcs_status mcs_probability_analysis(const mcs_pdag *pdag, const mcs_settings *se
 typedef struct mcs_batch mcs_batch;
 
 typedef struct mcs_batch_config {
-    uint32_t lds_bytes;       /* LDS per workgroup for a tile (0 = 64 KiB); raised
-                                 to fit the largest tree at one word, up to 160 KiB */
+    uint32_t lds_bytes;       /* LDS per workgroup of the fused launch (0 = 64 KiB) */
     uint32_t max_tile_words;  /* words per tile, power of two <= 64 (0 = 64) */
     uint32_t item_words;      /* largest work item in words (0 = 4096); smaller
                                  when the batch has too few tiles to fill the GPU */
     uint32_t use_graph;       /* 1: replay each pass as a captured hipGraph */
     uint32_t key0, key1;      /* Philox key (the reference key by default) */
     uint32_t workgroup_lanes; /* 64..1024, a multiple of 64 (0 = 1024) */
+    uint32_t min_fused_tile_words; /* narrowest LDS tile for the fused launch
+                                      (0 = 8; 1 = every tree that fits; 128 = none) */
 } mcs_batch_config;
 
 typedef struct mcs_batch_info {
     int32_t num_trees;
+    int32_t num_fused_trees;  /* trees in the persistent LDS launch */
+    int32_t num_layered_trees;/* trees in layered pipelines */
+    uint32_t layered_launches;/* layered kernel launches per pass */
     uint32_t num_items;       /* work items per pass */
     uint32_t grid;            /* persistent workgroups per launch */
     uint32_t lds_bytes;       /* LDS per workgroup */
@@ -285,7 +289,7 @@ mcs_status mcs_batch_wait(mcs_batch *batch);
 /* Tallies of one tree, in the order of its tally list (waits first). */
 mcs_status mcs_batch_collect(mcs_batch *batch, int32_t tree, mcs_tally *out, int32_t n_out);
 mcs_status mcs_batch_get_info(mcs_batch *batch, mcs_batch_info *out);
-/* Tile width (words) chosen for a tree. */
+/* Tile width (words) chosen for a tree; 0 for a tree in a layered pipeline. */
 mcs_status mcs_batch_tile_words(mcs_batch *batch, int32_t tree, uint32_t *out);
 /* Run n_passes with HIP events around each pass on the batch's stream;
  * kernel_ms receives the summed device time. Measurement hook. */

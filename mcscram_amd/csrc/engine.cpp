@@ -21,6 +21,7 @@
 #include <vector>
 
 #include "kernels.h"
+#include "launch_plan.h"
 #include "layering.h"
 #include "mcscram_amd.h"
 #include "status.h"
@@ -162,62 +163,21 @@ mcs_status build_engine(const mcs_pdag *pdag, const mcs_sample_shape *shape,
     eng->n_rows = uint64_t(eng->n_buf) * n_ev + plan.n_gates() + uint64_t(sch.n_aux);
 
     // Launch plan: one launch per schedule layer; launch 0 also carries the
-    // tally-only tasks of tallied basic events.
-    std::vector<mcs::GateDesc> gd;
-    std::vector<uint32_t> args0, args1;  // argument rows for event buffer 0 / 1
-    std::vector<double> layer_cost;
-    auto arg_row = [&](int32_t node, uint32_t buf) -> uint32_t {
-        return node < n_ev ? uint32_t(node + int64_t(buf) * n_ev) : uint32_t(eng->gate_row(node));
-    };
-    eng->launch_gate_begin.push_back(0);
-    for (size_t l = 0; l < sch.layers.size(); ++l) {
-        double cost = 0;
-        auto add_gate = [&](uint32_t out_row, uint32_t type, int32_t k, uint32_t flags,
-                            const std::vector<int32_t> &pos, const std::vector<int32_t> &neg) {
-            mcs::GateDesc d{};
-            d.out_row = out_row;
-            d.arg_begin = uint32_t(args0.size());
-            const uint32_t n_in = uint32_t(pos.size() + neg.size());
-            d.counts = n_in | uint32_t(pos.size()) << 16;
-            d.type_k_flags = type | (uint32_t(uint8_t(k)) << 8) | flags;
-            for (const auto *v : {&pos, &neg})
-                for (int32_t i : *v) {
-                    args0.push_back(arg_row(i, 0));
-                    args1.push_back(arg_row(i, 1));
-                }
-            gd.push_back(d);
-            cost += n_in + ((flags & mcs::kGateNoStore) ? 0 : 1);
-        };
-        for (const mcs::ExecGate &g : sch.layers[l]) {
-            const bool t = g.out < plan.n_nodes() && tallied[g.out];
-            add_gate(uint32_t(eng->gate_row(g.out)), g.type, g.k, t ? mcs::kGateTallied : 0u,
-                     g.pos_in, g.neg_in);
-        }
-        if (l == 0) {
-            for (int32_t e = 0; e < n_ev; ++e)
-                if (tallied[e])
-                    add_gate(uint32_t(e), MCS_NULL, 0, mcs::kGateTallied | mcs::kGateNoStore, {e}, {});
-        }
-        eng->launch_gate_begin.push_back(uint32_t(gd.size()));
-        layer_cost.push_back(cost);
-    }
+    // tally-only tasks of tallied basic events (slot = event id).
+    mcs::TreeLaunchPlan lp;
+    const mcs::RowMap rows{{0u, eng->n_buf == 2 ? uint32_t(n_ev) : 0u},
+                           uint32_t(eng->n_buf) * uint32_t(n_ev), 0u};
+    mcs::build_launch_plan(plan, sch, tallied, rows, lp);
+    const std::vector<mcs::GateDesc> &gd = lp.gates;
+    const std::vector<uint32_t> &args0 = lp.args[0], &args1 = lp.args[1];
+    eng->launch_gate_begin = lp.launch_begin;
     eng->n_args = args0.size();
     eng->arg_span = std::max<uint64_t>(args0.size(), 1);
 
     // Sampler tasks of the next pass, split over the launches in proportion
     // to each launch's gate traffic.
     const uint64_t samp_total = uint64_t(n_ev) * geo.n_chunks;
-    double cost_sum = 0;
-    for (double c : layer_cost) cost_sum += c;
-    eng->launch_samp_begin.push_back(0);
-    double acc = 0;
-    for (size_t l = 0; l < layer_cost.size(); ++l) {
-        acc += layer_cost[l];
-        const uint64_t end = (l + 1 == layer_cost.size() || cost_sum <= 0)
-                                 ? samp_total
-                                 : uint64_t(double(samp_total) * (acc / cost_sum));
-        eng->launch_samp_begin.push_back(std::max(end, eng->launch_samp_begin.back()));
-    }
+    eng->launch_samp_begin = mcs::split_in_proportion(lp.launch_cost, samp_total);
     for (uint32_t l = 0; l < eng->n_launches(); ++l) {
         mcs::PassLaunch pl{};
         pl.n_gates = eng->launch_gate_begin[l + 1] - eng->launch_gate_begin[l];

@@ -68,7 +68,8 @@ struct PassLaunch {
     const GateDesc *gates;
     const uint32_t *arg_rows;
     uint32_t n_gates;
-    uint32_t samp_iteration;
+    uint32_t samp_iteration;       // iteration sampled (when iteration_ptr is null)
+    const uint32_t *iteration_ptr; // else *iteration_ptr + samp_iteration
     const EventDesc *events;
     uint64_t samp_begin, samp_end;
     uint32_t samp_row0;

@@ -44,7 +44,9 @@ __device__ __forceinline__ void sampler_task(const PassLaunch &L, uint64_t *__re
     const uint32_t e = uint32_t(task / geo.n_chunks);
     const uint32_t chunk = uint32_t(task - uint64_t(e) * geo.n_chunks);
     const EventDesc d = L.events[e];
-    const uint32_t c3base = (L.samp_iteration + 1u) << 6;
+    const uint32_t iteration =
+        L.iteration_ptr ? *L.iteration_ptr + L.samp_iteration : L.samp_iteration;
+    const uint32_t c3base = (iteration + 1u) << 6;
     uint64_t *row = words + uint64_t(L.samp_row0 + e) * geo.stride;
     const uint64_t base = uint64_t(chunk) * geo.chunk_words;
     for (uint32_t off = threadIdx.x * 2u; off < geo.chunk_words; off += 2u * kLanes) {

@@ -57,11 +57,14 @@ class _Info(C.Structure):
 class _BatchConfig(C.Structure):
     _fields_ = [("lds_bytes", C.c_uint32), ("max_tile_words", C.c_uint32),
                 ("item_words", C.c_uint32), ("use_graph", C.c_uint32),
-                ("key0", C.c_uint32), ("key1", C.c_uint32), ("workgroup_lanes", C.c_uint32)]
+                ("key0", C.c_uint32), ("key1", C.c_uint32), ("workgroup_lanes", C.c_uint32),
+                ("min_fused_tile_words", C.c_uint32)]
 
 
 class _BatchInfo(C.Structure):
-    _fields_ = [("num_trees", C.c_int32), ("num_items", C.c_uint32), ("grid", C.c_uint32),
+    _fields_ = [("num_trees", C.c_int32), ("num_fused_trees", C.c_int32),
+                ("num_layered_trees", C.c_int32), ("layered_launches", C.c_uint32),
+                ("num_items", C.c_uint32), ("grid", C.c_uint32),
                 ("lds_bytes", C.c_uint32), ("workgroup_lanes", C.c_uint32),
                 ("iteration", C.c_uint32), ("device_bytes", C.c_uint64)]
 

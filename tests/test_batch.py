@@ -22,11 +22,6 @@ def _mixed_trees():
     return trees
 
 
-def test_batch_rejects_tree_too_large_for_lds():
-    with pytest.raises(EngineError, match="too large for one LDS tile"):
-        mc.Batch([mc.large_tree()], 1024)
-
-
 def test_batch_rejects_bad_config():
     with pytest.raises(EngineError, match="power of two"):
         mc.Batch([mc.small_tree(1)], 1024, max_tile_words=48)
@@ -57,21 +52,46 @@ def test_batch_matches_oracle_per_tree(gpu):
 
 @pytest.mark.gpu
 @pytest.mark.parametrize("cfg", [dict(use_graph=0), dict(lds_bytes=8192, item_words=64),
-                                 dict(max_tile_words=1), dict(max_tile_words=16, item_words=100000),
-                                 dict(workgroup_lanes=64), dict(workgroup_lanes=256, lds_bytes=150 << 10)])
+                                 dict(max_tile_words=1, min_fused_tile_words=1),
+                                 dict(max_tile_words=16, item_words=100000),
+                                 dict(workgroup_lanes=64), dict(workgroup_lanes=256, lds_bytes=150 << 10),
+                                 dict(min_fused_tile_words=128), dict(lds_bytes=4096)])
 def test_batch_configs_give_identical_tallies(gpu, cfg):
+    # Fused only, layered only and mixed batches; tile widths from 1 to 64.
     trees = _mixed_trees()[:5]
     widths = _check_batch(trees, [3000] * 5, n_pass=2, **cfg)
     if cfg.get("max_tile_words") == 1:
         assert set(widths) == {1}
+    if cfg.get("min_fused_tile_words") == 128:
+        assert set(widths) == {0}
 
 
 @pytest.mark.gpu
 def test_batch_uneven_random_trees(gpu):
     # Trees of the batched workload's family (gate count log-uniform, events
-    # 0.6 x gates), at a 2^20-trial prefix each.
-    trees = mc.batched_trees(seed=3, n_trees=6, gate_range=(50, 2000))
-    _check_batch(trees, [1 << 14] * len(trees))
+    # 0.6 x gates), at a 2^20-trial prefix each; the large ones go through
+    # the layered pipeline, the small ones through the fused launch.
+    trees = mc.batched_trees(seed=3, n_trees=8, gate_range=(50, 5000))
+    with mc.Batch(trees, 1 << 14) as b:
+        info = b.info()
+    assert info["num_fused_trees"] > 0 and info["num_layered_trees"] > 0
+    _check_batch(trees, [1 << 14] * len(trees), n_pass=2)
+
+
+@pytest.mark.gpu
+def test_batch_layered_pipeline_mixed_shapes(gpu):
+    # Two pipelines (two sample shapes), passes split across calls.
+    trees = _mixed_trees()
+    shapes = [1000, 2048, 1000, 2048, 1000, 2048, 1000]
+    tallies = [_all_nodes(t) for t in trees]
+    with mc.Batch(trees, shapes, tallies, min_fused_tile_words=128) as b:
+        b.pass_(1)
+        b.wait()
+        b.pass_(2)
+        got = [b.collect(i) for i in range(len(trees))]
+    for i, (pd, n_bp) in enumerate(zip(trees, shapes)):
+        want, _ = orc.run(pd, n_bp, n_iters=3, tally=tallies[i])
+        assert [t.num_one_bits for t in got[i]] == [int(x) for x in want], i
 
 
 @pytest.mark.gpu
