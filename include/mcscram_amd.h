@@ -295,6 +295,77 @@ mcs_status mcs_batch_tile_words(mcs_batch *batch, int32_t tree, uint32_t *out);
  * kernel_ms receives the summed device time. Measurement hook. */
 mcs_status mcs_batch_time_passes(mcs_batch *batch, uint32_t n_passes, double *kernel_ms);
 
+/*
+ * Uncertainty analysis: one tree, K parameter samples (basic-event
+ * probability vectors), each an independent plain Monte Carlo run. The
+ * reference's outer loop (UncertaintyAnalyzer::Sample,
+ * src/uncertainty_analysis.h:154-169) calls the probability calculator once
+ * per sample; its DirectEval engine ignores the sampled probabilities
+ * (src/mc/queue/kernel_builder.h:179-181), so there is no reference Monte
+ * Carlo counterpart. Here the samples share the tree's gate descriptors and
+ * run as one pipeline: sample k's gate launches sample the basic events of
+ * sample k + 1. Sample k draws from Philox key (key0 + sample_id[k], key1),
+ * or from (key0, key1) for every sample with common_random_numbers.
+ */
+typedef struct mcs_ua mcs_ua;
+
+typedef struct mcs_ua_config {
+    uint32_t key0, key1;             /* base key (the reference key by default) */
+    uint32_t common_random_numbers;  /* 1: every sample draws the same stream */
+} mcs_ua_config;
+
+void mcs_ua_config_default(mcs_ua_config *cfg);
+
+/* p_samples[k * pdag->n_vars + i] is variable i's probability (the order of
+ * pdag->var_index) in sample k; values outside [0, 1] are clamped as the
+ * reference clamps sampled deviates (src/uncertainty_analysis.cc:108-120).
+ * sample_ids may be null (sample k has id k). tally_index lists the nodes to
+ * count in every sample (n_tally = 0 counts the root). */
+mcs_status mcs_ua_create(const mcs_pdag *pdag, const mcs_sample_shape *shape, int32_t n_samples,
+                         const double *p_samples, const uint32_t *sample_ids,
+                         const int32_t *tally_index, int32_t n_tally, const mcs_ua_config *cfg,
+                         mcs_ua **out);
+mcs_status mcs_ua_destroy(mcs_ua *ua);
+/* Every sample advances by n_passes iterations (enqueued, not waited for). */
+mcs_status mcs_ua_run(mcs_ua *ua, uint32_t n_passes);
+/* ones[k * n_tally + j] = one bits of tally j in sample k; *total_bits the
+ * trials per sample so far. Waits first. */
+mcs_status mcs_ua_collect(mcs_ua *ua, uint64_t *ones, int64_t n_out, uint64_t *total_bits);
+/* Device time of n_passes over every sample (measurement hook). */
+mcs_status mcs_ua_time_run(mcs_ua *ua, uint32_t n_passes, double *kernel_ms);
+
+/*
+ * Surrogate network for uncertainty analysis (no counterpart in the
+ * reference): an MLP n_in -> hidden x 3 (GELU) -> 1 with BF16 weights and
+ * activations, FP32 accumulation on the matrix cores, mean-squared-error
+ * loss and Adam on FP32 master weights. Host arrays are FP32, row-major.
+ * Parameter order: W1 [n_in x hidden], b1, W2 [hidden x hidden], b2, W3, b3,
+ * w4 [hidden], b4 (the layer product is x W).
+ */
+typedef struct mcs_mlp mcs_mlp;
+
+/* hidden must be a multiple of 64; weights He-uniform from a SplitMix64 seed. */
+mcs_status mcs_mlp_create(int32_t n_in, int32_t hidden, uint64_t seed, mcs_mlp **out);
+mcs_status mcs_mlp_destroy(mcs_mlp *net);
+mcs_status mcs_mlp_num_params(mcs_mlp *net, int64_t *n);
+mcs_status mcs_mlp_get_params(mcs_mlp *net, float *out, int64_t n);
+/* Sets the FP32 master weights (and their BF16 copy); resets Adam's state. */
+mcs_status mcs_mlp_set_params(mcs_mlp *net, const float *in, int64_t n);
+mcs_status mcs_mlp_predict(mcs_mlp *net, const float *x, int64_t n_rows, float *y_out);
+/* Mean squared error of one batch (n_rows a multiple of 64) and, if grad is
+ * not null, its gradient in parameter order. */
+mcs_status mcs_mlp_loss_grad(mcs_mlp *net, const float *x, const float *y, int64_t n_rows,
+                             float *loss_out, float *grad_out);
+/* Adam over shuffled minibatches (batch a multiple of 64); epoch_loss[e]
+ * receives the mean training loss of epoch e (may be null). */
+mcs_status mcs_mlp_fit(mcs_mlp *net, const float *x, const float *y, int64_t n_rows, int32_t epochs,
+                       int32_t batch, float lr, uint64_t seed, float *epoch_loss);
+/* Test hook for the matrix-core product: c = A x B with A, B rounded to BF16;
+ * a is [M x K] (or [K x M] read transposed), b is [K x N] (or [N x K]).
+ * M, N multiples of 64, K of 32. */
+mcs_status mcs_mlp_gemm_test(const float *a, const float *b, int32_t M, int32_t N, int32_t K,
+                             int32_t trans_a, int32_t trans_b, float *c);
+
 #ifdef __cplusplus
 }
 #endif

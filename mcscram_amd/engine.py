@@ -115,6 +115,11 @@ def load_library():
         "mcs_batch_get_info": [vp, vp],
         "mcs_batch_tile_words": [vp, i32, vp],
         "mcs_batch_time_passes": [vp, u32, vp],
+        "mcs_ua_create": [vp, vp, i32, vp, vp, vp, i32, vp, vp],
+        "mcs_ua_destroy": [vp],
+        "mcs_ua_run": [vp, u32],
+        "mcs_ua_collect": [vp, vp, C.c_int64, vp],
+        "mcs_ua_time_run": [vp, u32, vp],
     }
     for name, args in sigs.items():
         fn = getattr(L, name)
@@ -126,6 +131,8 @@ def load_library():
     L.mcs_batch_config_default.restype = None
     L.mcs_tally_point_estimates.argtypes = [vp]
     L.mcs_tally_point_estimates.restype = None
+    L.mcs_ua_config_default.argtypes = [vp]
+    L.mcs_ua_config_default.restype = None
     L.mcs_normal_quantile_two_sided.argtypes = [C.c_double]
     L.mcs_normal_quantile_two_sided.restype = C.c_double
     _LIB = L

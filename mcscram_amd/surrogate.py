@@ -1,0 +1,130 @@
+"""Surrogate network for uncertainty analysis (ctypes mirror of mcs_mlp_*).
+
+An MLP from the vector of sampled basic-event parameters (standard-normal
+scores of their log probabilities) to the top event's logit, trained on
+pairs the GPU Monte Carlo engine generates. BF16 weights and activations,
+FP32 accumulation on the matrix cores, MSE loss, Adam. No counterpart in
+the reference.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from .engine import _check, _p, load_library
+
+_SIGS = {
+    "mcs_mlp_create": [C.c_int32, C.c_int32, C.c_uint64, C.c_void_p],
+    "mcs_mlp_destroy": [C.c_void_p],
+    "mcs_mlp_num_params": [C.c_void_p, C.c_void_p],
+    "mcs_mlp_get_params": [C.c_void_p, C.c_void_p, C.c_int64],
+    "mcs_mlp_set_params": [C.c_void_p, C.c_void_p, C.c_int64],
+    "mcs_mlp_predict": [C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p],
+    "mcs_mlp_loss_grad": [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p],
+    "mcs_mlp_fit": [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64, C.c_int32, C.c_int32,
+                    C.c_float, C.c_uint64, C.c_void_p],
+    "mcs_mlp_gemm_test": [C.c_void_p, C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_int32,
+                          C.c_int32, C.c_void_p],
+}
+
+
+def _lib():
+    L = load_library()
+    if not getattr(L, "_mlp_bound", False):
+        for name, args in _SIGS.items():
+            fn = getattr(L, name)
+            fn.argtypes = args
+            fn.restype = C.c_int32
+        L._mlp_bound = True
+    return L
+
+
+def _f32(a):
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+class Mlp:
+    """n_in -> hidden x 3 (GELU) -> 1 on the GPU."""
+
+    def __init__(self, n_in: int, hidden: int = 512, seed: int = 1):
+        L = _lib()
+        h = C.c_void_p()
+        _check(L.mcs_mlp_create(n_in, hidden, seed, C.byref(h)))
+        self._h = h
+        self.n_in, self.hidden = n_in, hidden
+        n = C.c_int64()
+        _check(L.mcs_mlp_num_params(h, C.byref(n)))
+        self.n_params = n.value
+
+    def close(self):
+        if getattr(self, "_h", None):
+            _lib().mcs_mlp_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def params(self) -> np.ndarray:
+        out = np.zeros(self.n_params, dtype=np.float32)
+        _check(_lib().mcs_mlp_get_params(self._h, _p(out), out.shape[0]))
+        return out
+
+    def set_params(self, p) -> None:
+        p = _f32(p)
+        _check(_lib().mcs_mlp_set_params(self._h, _p(p), p.shape[0]))
+
+    def predict(self, x) -> np.ndarray:
+        x = _f32(x)
+        out = np.zeros(x.shape[0], dtype=np.float32)
+        _check(_lib().mcs_mlp_predict(self._h, _p(x), x.shape[0], _p(out)))
+        return out
+
+    def loss_grad(self, x, y):
+        x, y = _f32(x), _f32(y)
+        loss = C.c_float()
+        g = np.zeros(self.n_params, dtype=np.float32)
+        _check(_lib().mcs_mlp_loss_grad(self._h, _p(x), _p(y), x.shape[0], C.byref(loss), _p(g)))
+        return loss.value, g
+
+    def fit(self, x, y, epochs: int = 10, batch: int = 256, lr: float = 1e-3, seed: int = 7):
+        x, y = _f32(x), _f32(y)
+        losses = np.zeros(epochs, dtype=np.float32)
+        _check(_lib().mcs_mlp_fit(self._h, _p(x), _p(y), x.shape[0], epochs, batch, lr, seed,
+                                  _p(losses)))
+        return losses
+
+
+def gemm_test(a, b, trans_a=False, trans_b=False):
+    """c = A @ B on the matrix cores (A, B rounded to BF16)."""
+    a, b = _f32(a), _f32(b)
+    M = a.shape[1] if trans_a else a.shape[0]
+    K = a.shape[0] if trans_a else a.shape[1]
+    N = b.shape[0] if trans_b else b.shape[1]
+    c = np.zeros((M, N), dtype=np.float32)
+    _check(_lib().mcs_mlp_gemm_test(_p(a), _p(b), M, N, K, int(trans_a), int(trans_b), _p(c)))
+    return c
+
+
+class Ensemble:
+    """Members trained on the same pairs from different initial weights and
+    shuffles; their disagreement routes a sample back to plain Monte Carlo.
+    Targets are standardised for training (mean / std of the training
+    logits) and predictions mapped back."""
+
+    def __init__(self, n_in: int, hidden: int = 512, members: int = 2, seed: int = 11):
+        self.members = [Mlp(n_in, hidden, seed + 101 * i) for i in range(members)]
+        self.mu, self.sd = 0.0, 1.0
+
+    def fit(self, x, y, epochs=10, batch=256, lr=1e-3, seed=5):
+        y = np.asarray(y, dtype=np.float64)
+        self.mu, self.sd = float(y.mean()), float(y.std() or 1.0)
+        yn = (y - self.mu) / self.sd
+        return [m.fit(x, yn, epochs, batch, lr, seed + 17 * i) for i, m in enumerate(self.members)]
+
+    def predict(self, x) -> np.ndarray:
+        """-> [members, rows] logits."""
+        return np.stack([m.predict(x).astype(np.float64) * self.sd + self.mu for m in self.members])

@@ -44,6 +44,8 @@ def lib():
             [C.c_int32, C.c_uint64, C.c_uint32, C.c_uint32, C.c_int32, C.c_void_p, C.c_void_p,
              C.c_void_p, C.c_int32]
         L.orc_run.restype = C.c_int32
+        L.orc_run_keyed.argtypes = L.orc_run.argtypes + [C.c_uint32, C.c_uint32]
+        L.orc_run_keyed.restype = C.c_int32
         L.orc_point_estimates.argtypes = [C.c_uint64, C.c_uint64, C.c_void_p]
         L.orc_max_threads.restype = C.c_int32
         _LIB = L
@@ -107,17 +109,20 @@ def layout(pd):
     return ev, depth, order[:n]
 
 
-def run(pd, n_bp, first_iter=1, n_iters=1, tally=None, want_words=False, threads=0):
+def run(pd, n_bp, first_iter=1, n_iters=1, tally=None, want_words=False, threads=0, key=None):
     """Monte Carlo on the CPU. Returns (ones per tallied index, words or None).
 
     words: [n_vars + n_gates, n_bp] uint64 of the last pass, rows = variable
-    slots then gate slots (unreachable rows are zero)."""
+    slots then gate slots (unreachable rows are zero). key: Philox key
+    (default: the reference's fixed key)."""
     keep, args = _pdag_args(pd)
     tally = np.ascontiguousarray(tally if tally is not None else [pd.root], dtype=np.int32)
     ones = np.zeros(tally.shape[0], dtype=np.uint64)
     words = np.zeros((pd.n_vars + pd.n_gates, n_bp), dtype=np.uint64) if want_words else None
-    rc = lib().orc_run(*args, int(n_bp), int(first_iter), int(n_iters), tally.shape[0], _p(tally),
-                       _p(ones), _p(words) if words is not None else None, int(threads))
+    key = key or (382307844, 293830103)
+    rc = lib().orc_run_keyed(*args, int(n_bp), int(first_iter), int(n_iters), tally.shape[0],
+                             _p(tally), _p(ones), _p(words) if words is not None else None,
+                             int(threads), int(key[0]) & 0xFFFFFFFF, int(key[1]) & 0xFFFFFFFF)
     if rc != 0:
         raise ValueError(lib().orc_last_error().decode())
     return ones, words

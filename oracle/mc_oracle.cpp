@@ -200,12 +200,13 @@ inline uint64_t threshold_of(double p) {
 }
 
 inline uint64_t sample_word(uint32_t pdag_idx, uint32_t event_idx, uint32_t sample_idx,
-                            uint32_t iteration, uint64_t threshold) {
+                            uint32_t iteration, uint64_t threshold, uint32_t key0 = kKey0,
+                            uint32_t key1 = kKey1) {
     uint64_t word = 0;
     for (uint32_t gen = 0; gen < 16; ++gen) {
         uint32_t x[4] = {pdag_idx + 1u, event_idx + 1u, sample_idx + 1u,
                          ((iteration + 1u) << 6) + gen};
-        philox_rounds(x, kKey0, kKey1);
+        philox_rounds(x, key0, key1);
         for (uint32_t j = 0; j < 4; ++j) {
             if (uint64_t(x[j]) < threshold) word |= uint64_t(1) << (4 * gen + j);
         }
@@ -299,12 +300,12 @@ int32_t orc_layout(int32_t n_vars, const int32_t *var_index, const double *var_p
 // words of the LAST pass, rows in the order [variables by slot, gates by slot]
 // (n_vars + n_gates rows of n_bp words; unreachable rows left untouched).
 // Returns 0 on success.
-int32_t orc_run(int32_t n_vars, const int32_t *var_index, const double *var_p,
-                int32_t n_gates, const int32_t *gate_index, const uint8_t *gate_type,
-                const int32_t *gate_k, const int64_t *arg_off, const int32_t *args,
-                int32_t root, uint64_t n_bp, uint32_t first_iter, uint32_t n_iters,
-                int32_t n_tally, const int32_t *tally_index, uint64_t *ones_out,
-                uint64_t *words_out, int32_t n_threads) {
+int32_t orc_run_keyed(int32_t n_vars, const int32_t *var_index, const double *var_p,
+                      int32_t n_gates, const int32_t *gate_index, const uint8_t *gate_type,
+                      const int32_t *gate_k, const int64_t *arg_off, const int32_t *args,
+                      int32_t root, uint64_t n_bp, uint32_t first_iter, uint32_t n_iters,
+                      int32_t n_tally, const int32_t *tally_index, uint64_t *ones_out,
+                      uint64_t *words_out, int32_t n_threads, uint32_t key0, uint32_t key1) {
     Graph g;
     g.n_vars = n_vars; g.var_index = var_index; g.var_p = var_p;
     g.n_gates = n_gates; g.gate_index = gate_index; g.gate_type = gate_type;
@@ -375,7 +376,7 @@ int32_t orc_run(int32_t n_vars, const int32_t *var_index, const double *var_p,
                 for (size_t e = 0; e < L.event_order.size(); ++e) {
                     const int32_t vs = L.event_order[e];
                     vword[vs] = sample_word(uint32_t(var_index[vs]), uint32_t(e), uint32_t(s),
-                                            iteration, thr[vs]);
+                                            iteration, thr[vs], key0, key1);
                 }
                 for (int32_t gs : L.gate_order) {
                     const auto &in = gin[gs];
@@ -403,6 +404,18 @@ int32_t orc_run(int32_t n_vars, const int32_t *var_index, const double *var_p,
         }
     }
     return 0;
+}
+
+// orc_run_keyed with the reference's fixed key.
+int32_t orc_run(int32_t n_vars, const int32_t *var_index, const double *var_p,
+                int32_t n_gates, const int32_t *gate_index, const uint8_t *gate_type,
+                const int32_t *gate_k, const int64_t *arg_off, const int32_t *args,
+                int32_t root, uint64_t n_bp, uint32_t first_iter, uint32_t n_iters,
+                int32_t n_tally, const int32_t *tally_index, uint64_t *ones_out,
+                uint64_t *words_out, int32_t n_threads) {
+    return orc_run_keyed(n_vars, var_index, var_p, n_gates, gate_index, gate_type, gate_k, arg_off,
+                         args, root, n_bp, first_iter, n_iters, n_tally, tally_index, ones_out,
+                         words_out, n_threads, kKey0, kKey1);
 }
 
 // Point estimates from raw counters: out = {mean, std_err, lo95, hi95, lo99, hi99}.

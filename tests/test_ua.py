@@ -1,0 +1,102 @@
+"""Uncertainty analysis: the lognormal parameter model and the statistics
+against their closed forms, and the GPU engine's per-sample Monte Carlo runs
+bit-exact against the oracle run with the same probabilities and key.
+"""
+import numpy as np
+import pytest
+
+import mcscram_amd as mc
+from mcscram_amd import ua
+from oracle import binding as orc
+from tests.models import small_random
+
+KEY = (382307844, 293830103)
+
+
+def test_lognormal_model_median_and_error_factor():
+    # ln p ~ N(ln m, (ln EF / z95)^2): the median is m and the 95th
+    # percentile is m * EF (src/expression/random_deviate.cc:91-108).
+    m = ua.LognormalModel(np.array([1e-3, 1e-4]), np.array([3.0, 10.0]))
+    p, z = m.sample(200_000, seed=1)
+    med = np.median(p, axis=0)
+    q95 = np.quantile(p, 0.95, axis=0)
+    assert np.allclose(med, [1e-3, 1e-4], rtol=0.02)
+    assert np.allclose(q95 / med, [3.0, 10.0], rtol=0.03)
+    assert np.allclose(np.log(p), m.mu + m.sigma * z)
+
+
+def test_sampled_probabilities_are_clamped():
+    m = ua.LognormalModel(np.array([0.5]), np.array([10.0]))
+    p, _ = m.sample(10_000, seed=2)
+    assert p.max() == 1.0 and p.min() > 0.0
+
+
+def test_statistics_formulas():
+    # mean, sigma with n/(n-1), EF = exp(1.96 sigma), CI of the mean
+    # (src/uncertainty_analysis.cc:145-150).
+    x = np.array([0.1, 0.2, 0.4, 0.3])
+    s = ua.statistics(x, n_quantiles=4)
+    assert s["mean"] == pytest.approx(0.25)
+    sd = np.std(x, ddof=1)
+    assert s["sigma"] == pytest.approx(sd)
+    assert s["error_factor"] == pytest.approx(np.exp(1.96 * sd))
+    assert s["ci95"][0] == pytest.approx(0.25 - 1.96 * sd / 2)
+    assert list(s["quantiles"]) == [0.25, 0.5, 0.75, 1.0]
+    assert s["quantiles"][1.0] == pytest.approx(0.4)
+
+
+def test_uncertainty_tree_shape():
+    pd, m = ua.uncertainty_tree()
+    assert pd.n_vars == 2000 and pd.n_gates == 3000
+    assert np.all((m.median >= 1e-5) & (m.median <= 1e-2))
+    assert np.all((m.error_factor >= 3) & (m.error_factor <= 10))
+
+
+def _oracle_sample(pd, p, n_bp, key, n_iters, tally):
+    return orc.run(pd.with_probabilities(p), n_bp, n_iters=n_iters, tally=tally, key=key)[0]
+
+
+@pytest.mark.gpu
+def test_ua_samples_bit_exact_against_oracle(gpu):
+    pd = small_random(31, n_be=40, n_gates=60, n_layers=6)
+    rng = np.random.default_rng(3)
+    K = 7
+    P = np.clip(pd.var_p[None, :] * np.exp(rng.standard_normal((K, pd.n_vars))), 0, 1)
+    P[2, 5] = 1.0
+    P[3, :] = 0.0
+    tally = [pd.root, int(pd.gate_index[3]), int(pd.var_index[0])]
+    ids = np.array([10, 11, 12, 500, 7, 8, 9], dtype=np.uint32)
+    with ua.UncertaintyEngine(pd, 1500, P, sample_ids=ids, tally=tally) as eng:
+        eng.run(1)
+        eng.run(1)
+        ones, total = eng.collect()
+    assert total == 2 * 1500 * 64
+    for k in range(K):
+        want = _oracle_sample(pd, P[k], 1500, ((KEY[0] + int(ids[k])) & 0xFFFFFFFF, KEY[1]), 2, tally)
+        assert [int(v) for v in ones[k]] == [int(v) for v in want], k
+
+
+@pytest.mark.gpu
+def test_ua_common_random_numbers(gpu):
+    pd = small_random(32, n_be=30, n_gates=40, n_layers=5)
+    P = np.stack([pd.var_p, pd.var_p, np.clip(pd.var_p * 2, 0, 1)])
+    with ua.UncertaintyEngine(pd, 777, P, common_random_numbers=True) as eng:
+        eng.run(1)
+        ones, _ = eng.collect()
+    assert ones[0, 0] == ones[1, 0]  # same stream, same probabilities
+    for k in range(3):
+        assert int(ones[k, 0]) == int(_oracle_sample(pd, P[k], 777, KEY, 1, [pd.root])[0])
+
+
+@pytest.mark.gpu
+def test_ua_single_sample_equals_engine(gpu):
+    # One sample with the reference key and the tree's own probabilities is
+    # the plain engine run.
+    pd = mc.small_tree(1)
+    with ua.UncertaintyEngine(pd, 4096, pd.var_p[None, :], key=(KEY[0], KEY[1]),
+                              common_random_numbers=True) as eng:
+        eng.run(2)
+        ones, _ = eng.collect()
+    with mc.LayerManager(pd, 4096) as lm:
+        lm.pass_(2)
+        assert int(ones[0, 0]) == lm.collect_tallies()[0].num_one_bits
