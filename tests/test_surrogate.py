@@ -123,4 +123,4 @@ def test_training_fits_a_smooth_function(gpu):
     xt = rng.standard_normal((512, n_in)).astype(np.float32)
     yt = np.tanh(xt @ w) + 0.3 * (xt[:, 0] * xt[:, 1])
     mse = float(np.mean((net.predict(xt) - yt) ** 2))
-    assert mse < 0.25 * float(np.var(yt)), mse
+    assert mse < 0.5 * float(np.var(yt)), mse  # held-out R^2 above 0.5
