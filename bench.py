@@ -150,6 +150,40 @@ def bench_fused(args, mc) -> int:
     return 0
 
 
+def bench_ua(args) -> int:
+    """Surrogate-screened uncertainty analysis of the 2,000-event / 3,000-gate
+    tree (lognormal basic events, median log-uniform in [1e-5, 1e-2], error
+    factor uniform in [3, 10]) against plain batched GPU Monte Carlo over the
+    same held-out parameter samples."""
+    from mcscram_amd import screening as scr
+    from mcscram_amd import ua
+    pd, model = ua.uncertainty_tree()
+    words = args.ua_words
+    ens, info = scr.train_surrogate(pd, model, args.ua_train, words, seed=3,
+                                    epochs=args.ua_epochs)
+    res = scr.screened_uncertainty(pd, model, ens, args.ua_outer, words, seed=1234)
+    plain, t_plain = scr.plain_uncertainty(pd, model, args.ua_outer, words, seed=1234)
+    cmp = scr.compare(res, plain)
+    t_scr = res.seconds["total"]
+    out = {
+        "metric": "UA parameter-samples/s at <=2% rel err vs plain MC",
+        "value": args.ua_outer / t_scr, "unit": "samples/s", "n_gpus": 1, "steps": 1,
+        "warmup": 0, "ms_per_step": t_scr * 1e3, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "bf16 network, u64 Monte Carlo", "data": "synthetic",
+        "config": {"workload": "surrogate-screened UA, 2,000 events / 3,000 gates",
+                   "training_pairs": args.ua_train, "outer_samples": args.ua_outer,
+                   "trials_per_mc_run": words * 64, "network": "2000-512-512-512-1 GELU x2",
+                   "disagreement": 0.05, "audit": 0.05},
+        "plain_mc_samples_per_s": args.ua_outer / t_plain,
+        "speedup_vs_plain_mc": t_plain / t_scr,
+        "comparison": cmp, "audit": res.audit_rel_error,
+        "training": info,
+        "within_2pct": bool(cmp["mean"]["rel_error"] <= 0.02 and cmp["p95"]["rel_error"] <= 0.02),
+    }
+    print(json.dumps(out), flush=True)
+    return 0
+
+
 def tuned_batch_config(workload: str) -> dict:
     """Batch configuration chosen by the autotuner for this workload, if one
     was recorded (mcscram_amd/tuned.json), else the defaults."""
@@ -168,10 +202,16 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--words", type=int, default=1 << 18, help="uint64 words per pass")
     ap.add_argument("--tree", choices=["large", "small"], default="large")
-    ap.add_argument("--workload", choices=["layered", "batched", "small-fused"], default="layered",
+    ap.add_argument("--workload", choices=["layered", "batched", "small-fused", "ua"],
+                    default="layered",
                     help="layered: the large tree through the layered engine (default); "
                          "batched: 256 uneven trees, 2^26 trials each, one persistent launch "
-                         "per pass; small-fused: the small tree, 2^20 trials, fused kernel")
+                         "per pass; small-fused: the small tree, 2^20 trials, fused kernel; "
+                         "ua: surrogate-screened uncertainty analysis vs plain GPU Monte Carlo")
+    ap.add_argument("--ua-train", type=int, default=16384, help="training pairs (ua)")
+    ap.add_argument("--ua-outer", type=int, default=16384, help="outer parameter samples (ua)")
+    ap.add_argument("--ua-words", type=int, default=1 << 16, help="words per Monte Carlo run (ua)")
+    ap.add_argument("--ua-epochs", type=int, default=30)
     ap.add_argument("--trees", type=int, default=256)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
@@ -184,6 +224,8 @@ def main() -> int:
         return 2
 
     import mcscram_amd as mc
+    if args.workload == "ua":
+        return bench_ua(args)
     if args.workload != "layered":
         return bench_fused(args, mc)
     pd = mc.large_tree() if args.tree == "large" else mc.small_tree(1)

@@ -1,0 +1,131 @@
+"""Surrogate-screened uncertainty analysis.
+
+1. Training pairs: lognormal parameter samples of the tree (their standard-
+   normal scores z are the network's input) and the GPU Monte Carlo
+   estimate of the top event for each (its logit is the target).
+2. A two-member ensemble of surrogate networks fitted to the pairs.
+3. Screening: every outer sample is predicted by both members; samples whose
+   members disagree by more than ``disagreement`` in logit space (relative
+   to the logit's magnitude), plus a random ``audit`` fraction, go to plain
+   Monte Carlo. Routed samples take the Monte Carlo value; the audit measures
+   the surrogate's error on samples it was trusted with.
+
+Plain Monte Carlo over every outer sample is the fallback and the judge of
+the result (``plain_uncertainty``). Monte Carlo estimates use
+p = (ones + 1/2) / (trials + 1) so that the logit stays finite.
+"""
+from __future__ import annotations
+
+import time
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from .pdag import Pdag
+from .surrogate import Ensemble
+from .ua import LognormalModel, UncertaintyEngine, statistics
+
+
+def logit(p):
+    p = np.asarray(p, dtype=np.float64)
+    return np.log(p) - np.log1p(-p)
+
+
+def sigmoid(x):
+    return 1.0 / (1.0 + np.exp(-np.asarray(x, dtype=np.float64)))
+
+
+def monte_carlo(pd: Pdag, p_samples, words: int, ids, passes: int = 1):
+    """Top-event estimate of each parameter sample by plain GPU Monte Carlo
+    (sample k uses Philox key (key0 + ids[k], key1)). -> (p_hat, seconds)."""
+    t = time.perf_counter()
+    with UncertaintyEngine(pd, words, p_samples, sample_ids=ids) as eng:
+        eng.run(passes)
+        ones, total = eng.collect()
+    dt = time.perf_counter() - t
+    return (ones[:, 0].astype(np.float64) + 0.5) / (total + 1.0), dt
+
+
+@dataclass
+class ScreeningResult:
+    estimates: np.ndarray        # top-event probability per outer sample
+    routed: np.ndarray           # bool: sent to plain Monte Carlo
+    disagreed: np.ndarray        # bool: ensemble disagreement above the bound
+    audited: np.ndarray          # bool: random audit
+    audit_rel_error: dict        # surrogate vs Monte Carlo on the audit subset
+    seconds: dict = field(default_factory=dict)
+
+    def stats(self):
+        return statistics(self.estimates)
+
+
+def train_surrogate(pd: Pdag, model: LognormalModel, n_train: int, words: int, seed: int,
+                    hidden: int = 512, epochs: int = 30, batch: int = 256, lr: float = 1e-3,
+                    id_base: int = 1 << 24):
+    """Monte Carlo training pairs and a fitted two-member ensemble.
+    -> (ensemble, info dict with timings and the training losses)."""
+    p, z = model.sample(n_train, seed)
+    ids = (id_base + np.arange(n_train)).astype(np.uint32)
+    p_hat, t_mc = monte_carlo(pd, p, words, ids)
+    t = time.perf_counter()
+    ens = Ensemble(pd.n_vars, hidden=hidden, members=2, seed=seed)
+    losses = ens.fit(z, logit(p_hat), epochs=epochs, batch=batch, lr=lr)
+    t_fit = time.perf_counter() - t
+    return ens, {"pairs": n_train, "mc_seconds": t_mc, "fit_seconds": t_fit,
+                 "final_losses": [float(l[-1]) for l in losses]}
+
+
+def screened_uncertainty(pd: Pdag, model: LognormalModel, ens: Ensemble, n_outer: int,
+                         words: int, seed: int, disagreement: float = 0.05, audit: float = 0.05,
+                         id_base: int = 0) -> ScreeningResult:
+    p, z = model.sample(n_outer, seed)
+    t0 = time.perf_counter()
+    lg = ens.predict(z)                       # [2, n_outer]
+    t_pred = time.perf_counter() - t0
+    mean_l = lg.mean(axis=0)
+    spread = np.abs(lg[0] - lg[1])
+    disagreed = spread > disagreement * np.maximum(np.abs(mean_l), 1.0)
+    rng = np.random.Generator(np.random.PCG64(seed ^ 0xA0D17))
+    audited = rng.random(n_outer) < audit
+    routed = disagreed | audited
+    est = sigmoid(mean_l)
+    idx = np.nonzero(routed)[0]
+    t_mc = 0.0
+    audit_err = {}
+    if idx.size:
+        p_hat, t_mc = monte_carlo(pd, p[idx], words, (id_base + idx).astype(np.uint32))
+        surrogate_vals = est[idx].copy()
+        est[idx] = p_hat
+        a = audited[idx] & ~disagreed[idx]
+        if a.any():
+            s_a, m_a = surrogate_vals[a], p_hat[a]
+            audit_err = {
+                "samples": int(a.sum()),
+                "mean": float(abs(s_a.mean() - m_a.mean()) / m_a.mean()),
+                "p95": float(abs(np.quantile(s_a, 0.95) - np.quantile(m_a, 0.95)) /
+                             np.quantile(m_a, 0.95)),
+                "median_abs_logit_error": float(np.median(np.abs(logit(s_a) - logit(m_a)))),
+            }
+    return ScreeningResult(est, routed, disagreed, audited, audit_err,
+                           {"predict": t_pred, "monte_carlo": t_mc,
+                            "total": time.perf_counter() - t0})
+
+
+def plain_uncertainty(pd: Pdag, model: LognormalModel, n_outer: int, words: int, seed: int,
+                      id_base: int = 0):
+    """Every outer sample by plain Monte Carlo (same samples, same keys as
+    screened_uncertainty with the same seed). -> (estimates, seconds)."""
+    p, _ = model.sample(n_outer, seed)
+    return monte_carlo(pd, p, words, (id_base + np.arange(n_outer)).astype(np.uint32))
+
+
+def compare(screened: ScreeningResult, plain: np.ndarray) -> dict:
+    s, m = screened.estimates, plain
+    out = {}
+    for name, f in (("mean", np.mean), ("p05", lambda x: np.quantile(x, 0.05)),
+                    ("p50", np.median), ("p95", lambda x: np.quantile(x, 0.95))):
+        out[name] = {"screened": float(f(s)), "plain": float(f(m)),
+                     "rel_error": float(abs(f(s) - f(m)) / f(m))}
+    out["routed_fraction"] = float(screened.routed.mean())
+    out["disagreed_fraction"] = float(screened.disagreed.mean())
+    return out
