@@ -159,10 +159,15 @@ def bench_ua(args) -> int:
     from mcscram_amd import ua
     pd, model = ua.uncertainty_tree()
     words = args.ua_words
+    say = lambda m: print(f"[ua] {m}", file=sys.stderr, flush=True)  # noqa: E731
+    say(f"training pairs: {args.ua_train} x {words * 64} trials")
     ens, info = scr.train_surrogate(pd, model, args.ua_train, words, seed=3,
                                     epochs=args.ua_epochs)
+    say(f"trained: {info}")
     res = scr.screened_uncertainty(pd, model, ens, args.ua_outer, words, seed=1234)
+    say(f"screened: {res.seconds}, routed {res.routed.mean():.3f}")
     plain, t_plain = scr.plain_uncertainty(pd, model, args.ua_outer, words, seed=1234)
+    say(f"plain Monte Carlo: {t_plain:.1f} s")
     cmp = scr.compare(res, plain)
     t_scr = res.seconds["total"]
     out = {
