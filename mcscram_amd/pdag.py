@@ -147,7 +147,8 @@ class SplitMix64:
 
 def random_layered_tree(seed: int, n_be: int, n_gates: int, n_layers: int,
                         p_range=(1e-4, 1e-1), fan_in=(2, 4), mix=(0.45, 0.45, 0.10),
-                        atleast_n=(3, 6), neg_frac=0.0, all_connectives=False) -> Pdag:
+                        atleast_n=(3, 6), neg_frac=0.0, all_connectives=False,
+                        top="orphans") -> Pdag:
     """Random layered fault tree with a single OR top gate.
 
     * Basic events: PDAG indices 2..n_be+1, p log-uniform in ``p_range``.
@@ -161,8 +162,13 @@ def random_layered_tree(seed: int, n_be: int, n_gates: int, n_layers: int,
       drawn from XOR (2 inputs), NOT and NULL (1 input), NAND and NOR.
     * Each input is complemented with probability ``neg_frac``.
     * Basic events no gate drew are appended to layer-1 gates (OR gates
-      first) so every basic event is reachable; gates no gate drew feed the
-      top OR gate.
+      first) so every basic event is reachable.
+    * ``top="orphans"``: every gate no other gate drew feeds the top OR gate.
+      ``top="last_layer"``: the top OR gate reads the last generator layer's
+      gates, and gates of lower layers that no gate drew become an extra
+      input of a random AND gate of a higher layer (any gate if the higher
+      layers hold no AND gate). Both keep every gate reachable; the second
+      keeps the top gate's fan-in, and so its probability, moderate.
 
     The reference's layer of a gate is its computed depth, which can be lower
     than the generator layer.
@@ -219,7 +225,22 @@ def random_layered_tree(seed: int, n_be: int, n_gates: int, n_layers: int,
         gates[gi][3].append(v)
         used.add(v)
 
-    top_args = [g[0] for g in gates if g[0] not in used]
+    if top == "last_layer":
+        last = max(layer_of)
+        for gi, g in enumerate(gates):
+            if layer_of[gi] == last or g[0] in used:
+                continue
+            higher = [j for j in range(len(gates)) if layer_of[j] > layer_of[gi]]
+            ands = [j for j in higher if gates[j][1] == AND and layer_of[j] < last] or \
+                [j for j in higher if gates[j][1] == AND] or higher
+            j = ands[rng.randint(0, len(ands) - 1)]
+            gates[j][3].append(g[0])
+            used.add(g[0])
+        top_args = [g[0] for gi, g in enumerate(gates) if layer_of[gi] == last]
+    elif top == "orphans":
+        top_args = [g[0] for g in gates if g[0] not in used]
+    else:
+        raise ValueError(f"top: {top}")
     if len(top_args) < 2:
         extra = [g[0] for g in gates[::-1] if g[0] not in top_args]
         top_args.extend(extra[:2 - len(top_args)])

@@ -118,11 +118,13 @@ class LognormalModel:
 
 
 def uncertainty_tree(seed: int = 4, n_be: int = 2000, n_gates: int = 3000):
-    """The small tree's family scaled up (2,000 events, 3,000 gates), with
-    lognormal uncertainty: median log-uniform in [1e-5, 1e-2], error factor
-    uniform in [3, 10]. -> (pdag with the medians as p, LognormalModel)."""
+    """The small tree's family scaled up (2,000 events, 3,000 gates in 8
+    layers, 45 % AND / 45 % OR / 10 % ATLEAST, fan-in 2-4, single OR top
+    gate over the last layer), with lognormal uncertainty: median
+    log-uniform in [1e-5, 1e-2], error factor uniform in [3, 10].
+    -> (pdag with the medians as p, LognormalModel)."""
     pd = random_layered_tree(seed, n_be, n_gates, 8, p_range=(1e-5, 1e-2), fan_in=(2, 4),
-                             mix=(0.45, 0.45, 0.10), atleast_n=(3, 6))
+                             mix=(0.45, 0.45, 0.10), atleast_n=(3, 6), top="last_layer")
     rng = SplitMix64(seed ^ 0x5EED)
     ef = np.array([3.0 + 7.0 * rng.uniform() for _ in range(n_be)])
     return pd, LognormalModel(pd.var_p.copy(), ef)
