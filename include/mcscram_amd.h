@@ -351,13 +351,23 @@ mcs_status mcs_mlp_num_params(mcs_mlp *net, int64_t *n);
 mcs_status mcs_mlp_get_params(mcs_mlp *net, float *out, int64_t n);
 /* Sets the FP32 master weights (and their BF16 copy); resets Adam's state. */
 mcs_status mcs_mlp_set_params(mcs_mlp *net, const float *in, int64_t n);
+/* Fixed (not trained) input map applied to every input column before the
+ * first layer: u_i = min(cap_i, exp(scale_i * x_i + shift_i)), in FP32, then
+ * rounded to BF16. With x_i the standard-normal score of ln p_i, scale_i =
+ * sigma_i and shift_i = mu_i - ln s_i, u_i is p_i / s_i (the probability on a
+ * per-event scale s_i), which the top event depends on far more linearly
+ * than on ln p_i. Each array has n_in entries; all three null clears the
+ * map (inputs then go to the first layer as given). */
+mcs_status mcs_mlp_set_input_map(mcs_mlp *net, const float *scale, const float *shift,
+                                 const float *cap);
 mcs_status mcs_mlp_predict(mcs_mlp *net, const float *x, int64_t n_rows, float *y_out);
 /* Mean squared error of one batch (n_rows a multiple of 64) and, if grad is
  * not null, its gradient in parameter order. */
 mcs_status mcs_mlp_loss_grad(mcs_mlp *net, const float *x, const float *y, int64_t n_rows,
                              float *loss_out, float *grad_out);
-/* Adam over shuffled minibatches (batch a multiple of 64); epoch_loss[e]
- * receives the mean training loss of epoch e (may be null). */
+/* Adam over shuffled minibatches (batch a multiple of 64) with a cosine
+ * learning-rate schedule: epoch e uses lr * (1 + cos(pi e / epochs)) / 2.
+ * epoch_loss[e] receives the mean training loss of epoch e (may be null). */
 mcs_status mcs_mlp_fit(mcs_mlp *net, const float *x, const float *y, int64_t n_rows, int32_t epochs,
                        int32_t batch, float lr, uint64_t seed, float *epoch_loss);
 /* Test hook for the matrix-core product: c = A x B with A, B rounded to BF16;

@@ -139,13 +139,21 @@ __global__ void gather_rows(const uint16_t *__restrict__ x, int32_t dp, const fl
     if (i < rows) yb[i] = y[idx[i]];
 }
 
-// FP32 [rows x n_in] -> BF16 [rows x dp] (zero columns beyond n_in).
+// FP32 [rows x n_in] -> BF16 [rows x dp] (zero columns beyond n_in). With
+// `map` ([scale | shift | cap], n_in each) column c becomes
+// min(cap_c, exp(scale_c x + shift_c)) before rounding.
 __global__ void to_bf16_padded(const float *__restrict__ x, int64_t rows, int32_t n_in, int32_t dp,
-                               uint16_t *__restrict__ out) {
+                               const float *__restrict__ map, uint16_t *__restrict__ out) {
     const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i >= rows * dp) return;
     const int64_t r = i / dp, c = i - r * dp;
-    out[i] = c < n_in ? to_bf16(x[r * n_in + c]) : uint16_t(0);
+    if (c >= n_in) {
+        out[i] = 0;
+        return;
+    }
+    float v = x[r * n_in + c];
+    if (map) v = fminf(map[2 * n_in + c], expf(fmaf(map[c], v, map[n_in + c])));
+    out[i] = to_bf16(v);
 }
 
 // z = c + b (kept for the backward pass), a = BF16(GELU(z)).
@@ -282,6 +290,7 @@ struct mcs_mlp {
             off_b4 = 0, n_dev = 0;
     float *p = nullptr, *g = nullptr, *m = nullptr, *v = nullptr;
     uint16_t *pb = nullptr;  // BF16 copy (same layout)
+    float *map = nullptr;    // input map [scale | shift | cap] or null
     int64_t adam_t = 0;
     // Workspace for a batch of `cap` rows.
     int32_t cap = 0;
@@ -297,7 +306,7 @@ struct mcs_mlp {
             (void)hipStreamSynchronize(stream);
         }
         for (void *x : {static_cast<void *>(p), static_cast<void *>(g), static_cast<void *>(m),
-                        static_cast<void *>(v), static_cast<void *>(pb)})
+                        static_cast<void *>(v), static_cast<void *>(pb), static_cast<void *>(map)})
             if (x) (void)hipFree(x);
         for (void *x : ws) (void)hipFree(x);
         if (stream) (void)hipStreamDestroy(stream);
@@ -414,7 +423,7 @@ mcs_status stage_rows(mcs_mlp *n, const float *x, const float *y, int64_t first,
     hipError_t e = hipMemcpyAsync(tmp, x + first * n->n_in, bytes, hipMemcpyHostToDevice, n->stream);
     if (e == hipSuccess) {
         hipLaunchKernelGGL(mcs::to_bf16_padded, dim3(mcs::blocks_for(int64_t(rows) * n->dp)), dim3(256),
-                           0, n->stream, tmp, int64_t(rows), n->n_in, n->dp, n->xb);
+                           0, n->stream, tmp, int64_t(rows), n->n_in, n->dp, n->map, n->xb);
         e = hipGetLastError();
     }
     if (e == hipSuccess && y)
@@ -542,6 +551,33 @@ mcs_status mcs_mlp_set_params(mcs_mlp *net, const float *in, int64_t n) {
     return copy_params(net, const_cast<float *>(in), n, false, nullptr);
 }
 
+mcs_status mcs_mlp_set_input_map(mcs_mlp *net, const float *scale, const float *shift,
+                                 const float *cap) {
+    if (!net) return fail(MCS_ERR_INVALID_ARGUMENT, "null network");
+    const int nulls = !scale + !shift + !cap;
+    if (nulls != 0 && nulls != 3)
+        return fail(MCS_ERR_INVALID_ARGUMENT, "input map: give scale, shift and cap, or none");
+    MCS_HIP(hipSetDevice(net->device));
+    MCS_HIP(hipStreamSynchronize(net->stream));
+    if (nulls == 3) {
+        if (net->map) MCS_HIP(hipFree(net->map));
+        net->map = nullptr;
+        return MCS_OK;
+    }
+    const size_t n = size_t(net->n_in);
+    std::vector<float> h(3 * n);
+    for (size_t c = 0; c < n; ++c) {
+        if (!std::isfinite(scale[c]) || !std::isfinite(shift[c]) || std::isnan(cap[c]))
+            return fail(MCS_ERR_INVALID_ARGUMENT, "input map: non-finite scale or shift, or NaN cap");
+        h[c] = scale[c];
+        h[n + c] = shift[c];
+        h[2 * n + c] = cap[c];
+    }
+    if (!net->map) MCS_HIP(hipMalloc(&net->map, h.size() * 4));
+    MCS_HIP(hipMemcpy(net->map, h.data(), h.size() * 4, hipMemcpyHostToDevice));
+    return MCS_OK;
+}
+
 mcs_status mcs_mlp_predict(mcs_mlp *net, const float *x, int64_t n_rows, float *y_out) {
     if (!net || !x || !y_out || n_rows < 0) return fail(MCS_ERR_INVALID_ARGUMENT, "null argument");
     MCS_HIP(hipSetDevice(net->device));
@@ -624,7 +660,7 @@ mcs_status mcs_mlp_fit(mcs_mlp *net, const float *x, const float *y, int64_t n_r
                            hipMemcpyHostToDevice, net->stream);
         if (e == hipSuccess) {
             hipLaunchKernelGGL(mcs::to_bf16_padded, dim3(mcs::blocks_for(r * net->dp)), dim3(256), 0,
-                               net->stream, tmp, r, net->n_in, net->dp, xall + f * net->dp);
+                               net->stream, tmp, r, net->n_in, net->dp, net->map, xall + f * net->dp);
             e = hipGetLastError();
         }
         if (e == hipSuccess) e = hipStreamSynchronize(net->stream);
@@ -645,7 +681,9 @@ mcs_status mcs_mlp_fit(mcs_mlp *net, const float *x, const float *y, int64_t n_r
     };
     std::vector<int32_t> perm(static_cast<size_t>(n_rows));
     std::vector<float> host_losses(static_cast<size_t>(steps));
+    const double pi = 3.14159265358979323846;
     for (int32_t ep = 0; ep < epochs && st == MCS_OK; ++ep) {
+        const float lr_ep = float(double(lr) * 0.5 * (1.0 + std::cos(pi * double(ep) / double(epochs))));
         for (int64_t i = 0; i < n_rows; ++i) perm[size_t(i)] = int32_t(i);
         for (int64_t i = n_rows - 1; i > 0; --i) {  // Fisher-Yates
             const int64_t j = int64_t(next() % uint64_t(i + 1));
@@ -660,7 +698,7 @@ mcs_status mcs_mlp_fit(mcs_mlp *net, const float *x, const float *y, int64_t n_r
                                net->yb);
             st = forward(net, batch, true);
             if (st == MCS_OK) st = backward(net, batch);
-            if (st == MCS_OK) st = adam_step(net, lr);
+            if (st == MCS_OK) st = adam_step(net, lr_ep);
             if (st == MCS_OK) {
                 hipLaunchKernelGGL(mcs::sum_vec, dim3(1), dim3(64), 0, net->stream, net->sq, batch,
                                    losses + sidx);

@@ -1,9 +1,15 @@
 """Surrogate-screened uncertainty analysis.
 
 1. Training pairs: lognormal parameter samples of the tree (their standard-
-   normal scores z are the network's input) and the GPU Monte Carlo
-   estimate of the top event for each (its logit is the target).
-2. A two-member ensemble of surrogate networks fitted to the pairs.
+   normal scores z, the normalised log probabilities, are the network's
+   input) and the GPU Monte Carlo estimate of the top event for each (its
+   logit is the target).
+2. A two-member ensemble of surrogate networks fitted to the pairs. Each
+   network first maps z_i to p_i / q_i, with q_i the event's 99th percentile
+   (``lognormal_input_map``): the top event is a sum of products of event
+   probabilities, nearly linear in p and far from linear in ln p, and on the
+   2,000-event uncertainty tree a linear fit of the logit explains 35 % of its
+   variance on p against 1 % on ln p.
 3. Screening: every outer sample is predicted by both members; samples whose
    members disagree by more than ``disagreement`` in logit space (relative
    to the logit's magnitude), plus a random ``audit`` fraction, go to plain
@@ -59,16 +65,37 @@ class ScreeningResult:
         return statistics(self.estimates)
 
 
+Z99 = 2.3263478740408408  # standard-normal 99th percentile
+
+
+def lognormal_input_map(model: LognormalModel):
+    """(scale, shift, cap) with min(cap, exp(scale z + shift)) = p / q99 for
+    p = min(1, exp(mu + sigma z)) and q99 = exp(mu + Z99 sigma)."""
+    scale = model.sigma
+    shift = -Z99 * model.sigma
+    cap = np.exp(-(model.mu + Z99 * model.sigma))
+    return scale, shift, cap
+
+
 def train_surrogate(pd: Pdag, model: LognormalModel, n_train: int, words: int, seed: int,
                     hidden: int = 512, epochs: int = 30, batch: int = 256, lr: float = 1e-3,
-                    id_base: int = 1 << 24):
-    """Monte Carlo training pairs and a fitted two-member ensemble.
+                    id_base: int = 1 << 24, chunk: int = 16384, progress=None):
+    """Monte Carlo training pairs (in runs of ``chunk`` samples, each followed
+    by ``progress(done, n_train)`` if given) and a fitted two-member ensemble.
     -> (ensemble, info dict with timings and the training losses)."""
     p, z = model.sample(n_train, seed)
     ids = (id_base + np.arange(n_train)).astype(np.uint32)
-    p_hat, t_mc = monte_carlo(pd, p, words, ids)
+    p_hat = np.empty(n_train)
+    t_mc = 0.0
+    for a in range(0, n_train, chunk):
+        b = min(n_train, a + chunk)
+        p_hat[a:b], dt = monte_carlo(pd, p[a:b], words, ids[a:b])
+        t_mc += dt
+        if progress:
+            progress(b, n_train)
     t = time.perf_counter()
-    ens = Ensemble(pd.n_vars, hidden=hidden, members=2, seed=seed)
+    ens = Ensemble(pd.n_vars, hidden=hidden, members=2, seed=seed,
+                   input_map=lognormal_input_map(model))
     losses = ens.fit(z, logit(p_hat), epochs=epochs, batch=batch, lr=lr)
     t_fit = time.perf_counter() - t
     return ens, {"pairs": n_train, "mc_seconds": t_mc, "fit_seconds": t_fit,

@@ -2,9 +2,11 @@
 
 An MLP from the vector of sampled basic-event parameters (standard-normal
 scores of their log probabilities) to the top event's logit, trained on
-pairs the GPU Monte Carlo engine generates. BF16 weights and activations,
-FP32 accumulation on the matrix cores, MSE loss, Adam. No counterpart in
-the reference.
+pairs the GPU Monte Carlo engine generates. A fixed input map turns each
+score back into the event's probability on a per-event scale before the
+first layer. BF16 weights and activations, FP32 accumulation on the matrix
+cores, MSE loss, Adam with a cosine schedule. No counterpart in the
+reference.
 """
 from __future__ import annotations
 
@@ -20,6 +22,7 @@ _SIGS = {
     "mcs_mlp_num_params": [C.c_void_p, C.c_void_p],
     "mcs_mlp_get_params": [C.c_void_p, C.c_void_p, C.c_int64],
     "mcs_mlp_set_params": [C.c_void_p, C.c_void_p, C.c_int64],
+    "mcs_mlp_set_input_map": [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p],
     "mcs_mlp_predict": [C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p],
     "mcs_mlp_loss_grad": [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p],
     "mcs_mlp_fit": [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64, C.c_int32, C.c_int32,
@@ -77,6 +80,17 @@ class Mlp:
         p = _f32(p)
         _check(_lib().mcs_mlp_set_params(self._h, _p(p), p.shape[0]))
 
+    def set_input_map(self, scale=None, shift=None, cap=None) -> None:
+        """Fixed map of each input column before the first layer:
+        u = min(cap, exp(scale * x + shift)); no arguments clears it."""
+        if scale is None and shift is None and cap is None:
+            _check(_lib().mcs_mlp_set_input_map(self._h, None, None, None))
+            return
+        arrs = [_f32(a) for a in (scale, shift, cap)]
+        if any(a.shape != (self.n_in,) for a in arrs):
+            raise ValueError(f"input map arrays must have {self.n_in} entries")
+        _check(_lib().mcs_mlp_set_input_map(self._h, *(_p(a) for a in arrs)))
+
     def predict(self, x) -> np.ndarray:
         x = _f32(x)
         out = np.zeros(x.shape[0], dtype=np.float32)
@@ -91,6 +105,8 @@ class Mlp:
         return loss.value, g
 
     def fit(self, x, y, epochs: int = 10, batch: int = 256, lr: float = 1e-3, seed: int = 7):
+        """Adam, cosine learning-rate schedule from lr down to 0 over the epochs.
+        -> mean training loss of each epoch."""
         x, y = _f32(x), _f32(y)
         losses = np.zeros(epochs, dtype=np.float32)
         _check(_lib().mcs_mlp_fit(self._h, _p(x), _p(y), x.shape[0], epochs, batch, lr, seed,
@@ -113,11 +129,16 @@ class Ensemble:
     """Members trained on the same pairs from different initial weights and
     shuffles; their disagreement routes a sample back to plain Monte Carlo.
     Targets are standardised for training (mean / std of the training
-    logits) and predictions mapped back."""
+    logits) and predictions mapped back. ``input_map`` (scale, shift, cap)
+    is set on every member (Mlp.set_input_map)."""
 
-    def __init__(self, n_in: int, hidden: int = 512, members: int = 2, seed: int = 11):
+    def __init__(self, n_in: int, hidden: int = 512, members: int = 2, seed: int = 11,
+                 input_map=None):
         self.members = [Mlp(n_in, hidden, seed + 101 * i) for i in range(members)]
         self.mu, self.sd = 0.0, 1.0
+        if input_map is not None:
+            for m in self.members:
+                m.set_input_map(*input_map)
 
     def fit(self, x, y, epochs=10, batch=256, lr=1e-3, seed=5):
         y = np.asarray(y, dtype=np.float64)

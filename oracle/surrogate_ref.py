@@ -41,6 +41,15 @@ def gelu_grad(x):
     return cdf + x * pdf
 
 
+def apply_input_map(x, scale, shift, cap):
+    """The network's fixed input map, FP32: min(cap, exp(scale x + shift))
+    per column (mcs_mlp_set_input_map)."""
+    f = np.float32
+    x = np.asarray(x, dtype=f)
+    v = np.exp(np.asarray(scale, dtype=f)[None, :] * x + np.asarray(shift, dtype=f)[None, :])
+    return np.minimum(np.asarray(cap, dtype=f)[None, :], v.astype(f))
+
+
 def unpack(params, n_in, h):
     p = np.asarray(params)
     o = 0

@@ -124,3 +124,45 @@ def test_training_fits_a_smooth_function(gpu):
     yt = np.tanh(xt @ w) + 0.3 * (xt[:, 0] * xt[:, 1])
     mse = float(np.mean((net.predict(xt) - yt) ** 2))
     assert mse < 0.5 * float(np.var(yt)), mse  # held-out R^2 above 0.5
+
+
+def test_lognormal_input_map_is_probability_over_q99():
+    # min(cap, exp(sigma z - Z99 sigma)) must equal min(1, exp(mu + sigma z))
+    # / exp(mu + Z99 sigma), clamped events included.
+    from mcscram_amd import screening as scr
+    from mcscram_amd import ua
+    m = ua.LognormalModel(np.array([1e-5, 1e-3, 0.3, 0.9]), np.array([3.0, 10.0, 10.0, 5.0]))
+    p, z = m.sample(5000, seed=4)
+    assert (p == 1.0).any()
+    scale, shift, cap = scr.lognormal_input_map(m)
+    u = np.minimum(cap[None, :], np.exp(scale[None, :] * z + shift[None, :]))
+    q99 = np.exp(m.mu + scr.Z99 * m.sigma)
+    assert np.allclose(u, p / q99[None, :], rtol=1e-12)
+    # About 1 % of the unclamped draws exceed their 99th percentile.
+    assert abs((u[:, :2] > 1.0).mean() - 0.01) < 0.004
+
+
+@pytest.mark.gpu
+def test_input_map_prediction_and_gradient(gpu):
+    from mcscram_amd.surrogate import Mlp
+    n_in, h = 70, 64
+    rng = np.random.default_rng(12)
+    scale = (0.5 + rng.random(n_in)).astype(np.float32)
+    shift = (-2.0 * rng.random(n_in)).astype(np.float32)
+    cap = np.exp(rng.random(n_in) * 3).astype(np.float32)
+    net = Mlp(n_in, h, seed=5)
+    p = net.params()
+    x = rng.standard_normal((128, n_in)).astype(np.float32)
+    y = rng.standard_normal(128).astype(np.float32)
+    net.set_input_map(scale, shift, cap)
+    u = ref.apply_input_map(x, scale, shift, cap)
+    assert np.allclose(net.predict(x), ref.predict(p, u, n_in, h), rtol=1e-2, atol=1e-3)
+    loss, g = net.loss_grad(x, y)
+    rloss, rg = ref.loss_and_grad(p, u, y, n_in, h)
+    assert abs(loss - rloss) <= 1e-3 * max(1.0, rloss)
+    assert np.linalg.norm(g - rg) <= 2e-2 * np.linalg.norm(rg)
+    # Cleared map: the raw inputs again.
+    net.set_input_map()
+    assert np.allclose(net.predict(x), ref.predict(p, x, n_in, h), rtol=1e-2, atol=1e-3)
+    with pytest.raises(Exception):
+        net.set_input_map(scale, shift, None)
