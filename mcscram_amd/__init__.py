@@ -10,3 +10,4 @@ from .pdag import (AND, ATLEAST, NAND, NOR, NOT, NULL, OR, XOR, Pdag,  # noqa: F
 from .engine import (Batch, LayerManager, Settings, Tally, library_path, load_library,  # noqa: F401
                      normal_quantile_two_sided, pdag_event_order, pdag_gate_depths,
                      plan_passes, probability_analysis)
+from . import mef  # noqa: F401  (Open-PSA MEF -> PDAG at compilation level 0)

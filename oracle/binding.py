@@ -48,6 +48,10 @@ def lib():
         L.orc_run_keyed.restype = C.c_int32
         L.orc_point_estimates.argtypes = [C.c_uint64, C.c_uint64, C.c_void_p]
         L.orc_max_threads.restype = C.c_int32
+        L.orc_bdd_probability.argtypes = [C.c_int32, C.c_void_p, C.c_void_p, C.c_int32] + \
+            [C.c_void_p] * 5 + [C.c_int32, C.c_uint64, C.c_void_p]
+        L.orc_bdd_probability.restype = C.c_double
+        L.orc_bdd_last_error.restype = C.c_char_p
         _LIB = L
     return _LIB
 
@@ -132,6 +136,17 @@ def point_estimates(ones: int, total: int):
     out = np.zeros(6, dtype=np.float64)
     lib().orc_point_estimates(int(ones), int(total), _p(out))
     return out
+
+
+def bdd_probability(pd, max_nodes: int = 20_000_000):
+    """Exact P(root) by a BDD -> (probability, BDD nodes); probability is
+    None when the BDD would exceed max_nodes."""
+    keep, args = _pdag_args(pd)
+    nodes = C.c_uint64()
+    p = lib().orc_bdd_probability(*args, int(max_nodes), C.byref(nodes))
+    if p == -2.0:
+        raise ValueError(lib().orc_bdd_last_error().decode())
+    return (None if p < 0 else p), int(nodes.value)
 
 
 def max_threads() -> int:
