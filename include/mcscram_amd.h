@@ -204,7 +204,37 @@ typedef struct mcs_settings {
     double ci_rel_margin;      /* relative half-width target, default 0.001 */
     uint32_t max_passes;       /* safety cap for auto mode (0 = 1<<20) */
     uint32_t key0, key1;       /* Philox key (reference key by default) */
+    uint32_t ci_policy;        /* early-stop rule: MCS_CI_BAYES (reference
+                                  default) or MCS_CI_WALD (src/settings.h:51, 426) */
 } mcs_settings;
+
+/* Stopping rules of the reference's convergence policy
+ * (src/mc/scheduler/convergence_policy.h; numbering of src/settings.h:51). */
+enum { MCS_CI_BAYES = 0, MCS_CI_WALD = 1 };
+
+/* Convergence state of one tally: current and target half-widths in linear
+ * and log10 space, and the trials each target needs. Mirrors the linear and
+ * log10 fields the reference's policy writes into stats::tally. */
+typedef struct mcs_convergence {
+    double epsilon, target_epsilon;
+    uint64_t target_trials;
+    double epsilon_log10, target_epsilon_log10;
+    uint64_t target_trials_log10;
+} mcs_convergence;
+
+/* One policy update for a tally of ones true trials out of total at normal
+ * quantile z and relative margin rel_margin. Replaces
+ * detail::update_wald / detail::update_bayes
+ * (src/mc/scheduler/convergence_policy.h:28-61, 66-143). */
+mcs_status mcs_convergence_update(uint32_t ci_policy, uint64_t ones, uint64_t total,
+                                  double rel_margin, double z, mcs_convergence *out);
+
+/* Quantile of the Beta(a, b) distribution: x with I_x(a, b) = q. The
+ * reference calls Boost.Math's quantile(beta_distribution) for the Bayes
+ * policy (src/mc/scheduler/convergence_policy.h:97-99); this one is written
+ * from the regularized incomplete beta function. Returns NaN for a, b <= 0
+ * or q outside [0, 1]. */
+double mcs_beta_quantile(double a, double b, double q);
 
 /* Two-sided standard-normal quantile z with P(|Z| <= z) = confidence
  * (Acklam's inverse normal; the reference's src/mc/stats/ci_utils.h:30-92). */
@@ -222,8 +252,8 @@ mcs_status mcs_plan_passes(uint64_t num_trials, const mcs_sample_shape *shape,
 /*
  * Entry point of the path: build the engine, run burn-in and then passes
  * until the fixed trial count is reached (or, with num_trials == 0, until
- * the root's CI half-width meets ci_rel_margin), collect tallies and return
- * the root's mean. Replaces ProbabilityAnalyzer<mc::DirectEval>::
+ * every tally's half-width meets ci_rel_margin under ci_policy, in linear and
+ * log10 space), collect tallies and return the root's mean. Replaces ProbabilityAnalyzer<mc::DirectEval>::
  * CalculateTotalProbability / ComputeTallies
  * (src/mc/core/probability_analysis.cc:88-209) and
  * convergence_controller::run_to_convergence (convergence_controller.h:371-397).

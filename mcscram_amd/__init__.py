@@ -7,7 +7,8 @@ that needs it fails loudly if it was not built (python -m mcscram_amd.build).
 """
 from .pdag import (AND, ATLEAST, NAND, NOR, NOT, NULL, OR, XOR, Pdag,  # noqa: F401
                    batched_trees, large_tree, random_layered_tree, small_tree)
-from .engine import (Batch, LayerManager, Settings, Tally, library_path, load_library,  # noqa: F401
+from .engine import (Batch, LayerManager, Settings, Tally, beta_quantile,  # noqa: F401
+                     convergence_update, library_path, load_library,
                      normal_quantile_two_sided, pdag_event_order, pdag_gate_depths,
                      plan_passes, probability_analysis)
 from . import mef  # noqa: F401  (Open-PSA MEF -> PDAG at compilation level 0)

@@ -533,6 +533,7 @@ void mcs_settings_default(mcs_settings *s) {
     s->max_passes = 0;
     s->key0 = MCS_REFERENCE_KEY0;
     s->key1 = MCS_REFERENCE_KEY1;
+    s->ci_policy = MCS_CI_BAYES;  // src/settings.h:426
 }
 
 mcs_status mcs_plan_passes(uint64_t num_trials, const mcs_sample_shape *shape,

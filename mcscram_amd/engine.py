@@ -72,7 +72,17 @@ class _BatchInfo(C.Structure):
 class _Settings(C.Structure):
     _fields_ = [("num_trials", C.c_uint64), ("burn_in_trials", C.c_uint64),
                 ("ci_confidence", C.c_double), ("ci_rel_margin", C.c_double),
-                ("max_passes", C.c_uint32), ("key0", C.c_uint32), ("key1", C.c_uint32)]
+                ("max_passes", C.c_uint32), ("key0", C.c_uint32), ("key1", C.c_uint32),
+                ("ci_policy", C.c_uint32)]
+
+
+class _Convergence(C.Structure):
+    _fields_ = [("epsilon", C.c_double), ("target_epsilon", C.c_double),
+                ("target_trials", C.c_uint64), ("epsilon_log10", C.c_double),
+                ("target_epsilon_log10", C.c_double), ("target_trials_log10", C.c_uint64)]
+
+
+CI_POLICIES = {"bayes": 0, "wald": 1}  # src/settings.h:51-53
 
 
 def library_path() -> str:
@@ -135,6 +145,9 @@ def load_library():
     L.mcs_ua_config_default.restype = None
     L.mcs_normal_quantile_two_sided.argtypes = [C.c_double]
     L.mcs_normal_quantile_two_sided.restype = C.c_double
+    L.mcs_beta_quantile.argtypes = [C.c_double, C.c_double, C.c_double]
+    L.mcs_beta_quantile.restype = C.c_double
+    L.mcs_convergence_update.argtypes = [u32, u64, u64, C.c_double, C.c_double, vp]
     _LIB = L
     return L
 
@@ -288,10 +301,14 @@ class Settings:
     ci_rel_margin: float = 0.001
     max_passes: int = 0
     key: tuple = MCS_REFERENCE_KEY
+    ci_policy: str = "bayes"  # src/settings.h:426
 
     def _c(self) -> _Settings:
+        if self.ci_policy not in CI_POLICIES:
+            raise ValueError(f"ci_policy must be one of {sorted(CI_POLICIES)}")
         return _Settings(self.num_trials, self.burn_in_trials, self.ci_confidence,
-                         self.ci_rel_margin, self.max_passes, self.key[0], self.key[1])
+                         self.ci_rel_margin, self.max_passes, self.key[0], self.key[1],
+                         CI_POLICIES[self.ci_policy])
 
 
 def probability_analysis(pdag: Pdag, settings: Settings, shape, observe=()):
@@ -339,6 +356,20 @@ def pdag_gate_depths(pdag: Pdag) -> np.ndarray:
 
 def normal_quantile_two_sided(confidence: float) -> float:
     return load_library().mcs_normal_quantile_two_sided(confidence)
+
+
+def beta_quantile(a: float, b: float, q: float) -> float:
+    """Quantile of Beta(a, b) (host only)."""
+    return load_library().mcs_beta_quantile(a, b, q)
+
+
+def convergence_update(policy: str, ones: int, total: int, rel_margin: float, z: float) -> dict:
+    """One tally's stopping-rule update (host only): current and target
+    half-widths and trial counts in linear and log10 space."""
+    out = _Convergence()
+    _check(load_library().mcs_convergence_update(CI_POLICIES[policy], ones, total, rel_margin, z,
+                                                 C.byref(out)))
+    return {f: getattr(out, f) for f, _ in _Convergence._fields_}
 
 
 class Batch:

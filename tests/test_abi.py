@@ -104,6 +104,8 @@ def test_settings_defaults_are_the_reference_defaults():
     assert (cs.num_trials, cs.burn_in_trials, cs.ci_confidence, cs.ci_rel_margin) == \
         (0, 1 << 20, 0.99, 0.001)
     assert (cs.key0, cs.key1) == (382307844, 293830103)
+    # Bayes is the reference's default stopping rule (src/settings.h:426).
+    assert cs.ci_policy == 0 and s.ci_policy == "bayes"
 
 
 def test_generators_are_deterministic_and_sized():

@@ -147,14 +147,38 @@ def test_pass_wait_collect_intervals(gpu):
     assert ta == tb
 
 
-def test_probability_analysis_early_stop(gpu):
+@pytest.mark.parametrize("policy", ["bayes", "wald"])
+def test_probability_analysis_early_stop(gpu, policy):
     pd, exact = core_models()["abc"]
-    settings = mc.Settings(num_trials=0, burn_in_trials=1 << 16, ci_rel_margin=0.01)
+    settings = mc.Settings(num_trials=0, burn_in_trials=1 << 16, ci_rel_margin=0.01,
+                           ci_policy=policy)
     p, tallies, passes = mc.probability_analysis(pd, settings, 1 << 12)
     t = tallies[0]
-    half = 2.5758293035489004 * t.std_err
-    assert half <= 0.01 * p * 1.000001
     assert abs(p - exact) <= 3.2905 * t.std_err
+    # Replay the controller on the host over the same passes: burn-in to
+    # 2^16 trials, then one pass at a time until both targets have been met
+    # (src/mc/scheduler/convergence_controller.h:294-397, 471-482).
+    z = mc.normal_quantile_two_sided(0.99)
+    eps_ok = log_ok = False
+    n = 0
+    with mc.LayerManager(pd, 1 << 12) as eng:
+        while n * (1 << 18) < (1 << 16):
+            r = eng.pass_wait_collect(1)[0]
+            n += 1
+
+        def update(r):
+            c = mc.convergence_update(policy, r.num_one_bits, r.total_bits, 0.01, z)
+            ok = c["epsilon"] > 0 and c["epsilon"] <= 0.01 * max(r.mean, 1e-12)
+            return ok, c["epsilon_log10"] > 0 and c["epsilon_log10"] <= 0.01
+
+        a, b = update(r)
+        eps_ok, log_ok = eps_ok or a, log_ok or b
+        while not (eps_ok and log_ok):
+            r = eng.pass_wait_collect(1)[0]
+            n += 1
+            a, b = update(r)
+            eps_ok, log_ok = eps_ok or a, log_ok or b
+    assert (passes, p) == (n, r.mean)
 
 
 def test_event_order_on_engine_matches_host_plan(gpu):
