@@ -100,6 +100,8 @@ typedef struct mcs_engine_info {
     uint64_t device_bytes;     /* device memory owned by the engine */
     int32_t event_buffers;     /* 2: the next pass is sampled during this one */
     uint32_t launches_per_pass;/* kernel launches of a pass in steady state */
+    int32_t fused_gates;       /* gates evaluated inside their only consumer's
+                                  task: their words never reach device memory */
 } mcs_engine_info;
 
 /* Message of the last failed call on this thread ("" if none). */

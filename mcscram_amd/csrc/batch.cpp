@@ -160,6 +160,7 @@ mcs_status enqueue_layered(mcs_batch *b, uint32_t it) {
             for (size_t l = 0; l + 1 < tr.lp.launch_begin.size(); ++l) {
                 mcs::PassLaunch pl = base;
                 pl.gates = g.d_gates + tr.gate_begin + tr.lp.launch_begin[l];
+                pl.tree_gates = g.d_gates + tr.gate_begin;
                 pl.arg_rows = g.d_args[area] + tr.args_begin;
                 pl.n_gates = tr.lp.launch_begin[l + 1] - tr.lp.launch_begin[l];
                 pl.events = g.d_events + nx.ev_begin;
@@ -471,7 +472,11 @@ mcs_status build_batch(const mcs_pdag *pdags, int32_t n_trees, const mcs_sample_
             const uint32_t gate_rows = uint32_t(th.n_nodes - n_ev);
             LayeredGroup::Tree tr;
             const mcs::RowMap rm{{0u, g.max_ev}, 2u * g.max_ev, 2u * g.max_ev + gate_rows};
-            mcs::build_launch_plan(th.plan, th.sch, th.tallied, rm, tr.lp);
+            // The fused LDS path keeps th.sch whole; the layered path fuses
+            // single-consumer gates into their consumers' tasks.
+            mcs::Schedule fsch = th.sch;
+            mcs::fuse_single_consumer(fsch, th.plan, kBatchMaxFanIn);
+            mcs::build_launch_plan(th.plan, fsch, th.tallied, rm, tr.lp);
             tr.gate_begin = gd.size();
             tr.args_begin = a0.size();
             tr.ev_begin = evs.size();

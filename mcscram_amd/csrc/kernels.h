@@ -47,6 +47,11 @@ inline EventDesc make_event_desc(int32_t pdag_index, uint32_t event_idx, double 
 constexpr uint32_t kGateTallied = 1u << 16;  // popcount into ones[out_row]
 constexpr uint32_t kGateNoStore = 1u << 17;  // tally only (basic-event tallies)
 
+// An argument entry with this bit set is not a row: the low bits index the
+// descriptor of a fused gate (in PassLaunch::tree_gates) whose word the task
+// computes in registers.
+constexpr uint32_t kInlineArg = 1u << 31;
+
 struct GateDesc {          // 16 bytes, one per gate, evaluation order
     uint32_t out_row;      // row written; also the tally slot
     uint32_t arg_begin;    // into the argument-row array
@@ -66,6 +71,7 @@ struct LaunchGeometry {
 // pairs) for the pass whose basic events go to rows samp_row0 + e.
 struct PassLaunch {
     const GateDesc *gates;
+    const GateDesc *tree_gates;    // base of kInlineArg references
     const uint32_t *arg_rows;
     uint32_t n_gates;
     uint32_t samp_iteration;       // iteration sampled (when iteration_ptr is null)

@@ -6,6 +6,9 @@
 //   gate tasks     one gate of the current layer over a chunk of words:
 //                  16-byte loads of each input row, bitwise fold, bit-sliced
 //                  counter for ATLEAST, popcount tally fused at production.
+//                  A task also computes, in registers, the gates fused into
+//                  it (gates read by this gate alone): their words never
+//                  reach HBM and their tallies are counted here.
 //   sampler tasks  Philox4x32-10 Bernoulli words of one basic event over a
 //                  chunk of words; lanes own consecutive words so every
 //                  per-event value sits in SGPRs and stores are coalesced.
@@ -33,6 +36,18 @@ __device__ __forceinline__ uint32_t live_popcount(uint64_t w0, uint64_t w1, uint
     return (s < n_bp ? __popcll(w0) : 0u) + (s + 1 < n_bp ? __popcll(w1) : 0u);
 }
 
+// Row words are streamed: a row is read by a handful of gates spread over the
+// whole pass, long after it has left the caches, so loads and stores are
+// non-temporal (measured 2 % faster per pass on the large tree).
+typedef unsigned long long u64x2 __attribute__((ext_vector_type(2)));
+
+__device__ __forceinline__ void store_pair(uint64_t *p, uint64_t x, uint64_t y) {
+    u64x2 v;
+    v.x = x;
+    v.y = y;
+    __builtin_nontemporal_store(v, reinterpret_cast<u64x2 *>(p));
+}
+
 __device__ __forceinline__ void tally_wave(unsigned long long *ones, uint32_t slot, uint32_t pc) {
     pc = wave_sum(pc);
     if ((threadIdx.x & 63u) == 0 && pc) atomicAdd(ones + slot, (unsigned long long)pc);
@@ -58,7 +73,7 @@ __device__ __forceinline__ void sampler_task(const PassLaunch &L, uint64_t *__re
         } else {
             w0 = w1 = (d.mode == kEventAlways) ? ~0ull : 0ull;
         }
-        *reinterpret_cast<ulonglong2 *>(row + s) = make_ulonglong2(w0, w1);
+        store_pair(row + s, w0, w1);
     }
 }
 
@@ -67,43 +82,120 @@ struct Pair {
 };
 
 __device__ __forceinline__ Pair load_pair(const uint64_t *__restrict__ p) {
-    const ulonglong2 v = *reinterpret_cast<const ulonglong2 *>(p);
+    const u64x2 v = __builtin_nontemporal_load(reinterpret_cast<const u64x2 *>(p));
     return {v.x, v.y};
 }
 
-// AND / OR / XOR families over K pairs per lane (lane words off + 512 k):
-// inputs two at a time so 2K 16-byte loads are in flight per lane.
-template <int K, int Op>  // Op: 0 AND, 1 OR, 2 XOR
+// Inputs of a task. A row input is a 16-byte load per pair; with kFused an
+// argument may instead reference a fused gate (kInlineArg), whose word the
+// task computes in registers from that gate's own row inputs and counts in
+// that gate's tally. Fused gates never reference fused gates.
+struct Inputs {
+    const PassLaunch *L;
+    const uint64_t *__restrict__ words;
+    unsigned long long *__restrict__ ones;
+    const LaunchGeometry *geo;
+};
+
+template <int K, bool kFused>
+__device__ __forceinline__ void eval_gate(const GateDesc &d, const uint32_t *__restrict__ a,
+                                          const Inputs &in, uint64_t s0, Pair (&r)[K]);
+
+template <int K, int Op, bool kFused>
 __device__ __forceinline__ void fold_simple(Pair (&r)[K], const uint32_t *__restrict__ a,
-                                            uint32_t n_in, uint32_t n_pos,
-                                            const uint64_t *__restrict__ words, uint64_t stride,
+                                            uint32_t n_in, uint32_t n_pos, const Inputs &in,
+                                            uint64_t s0);
+
+// Word pairs k = 0..K-1 (lane words s0 + 512 k) of a fused gate, tallied.
+// Fused gates are AND/OR/XOR/NAND/NOR/NOT/NULL over rows.
+template <int K>
+__device__ __forceinline__ void eval_fused(uint32_t ref, const Inputs &in, uint64_t s0,
+                                           Pair (&v)[K]) {
+    const GateDesc c = in.L->tree_gates[ref & ~kInlineArg];
+    const uint32_t *ca = in.L->arg_rows + c.arg_begin;
+    const uint32_t n_in = c.counts & 0xFFFFu, n_pos = c.counts >> 16;
+    const uint32_t type = c.type_k_flags & 0xFFu;
+    if (type == 4u || type == 7u) {  // NOT, NULL: the last input
+#pragma unroll
+        for (int k = 0; k < K; ++k) v[k] = {0, 0};
+        if (n_in) {
+            const uint64_t *p = in.words + uint64_t(ca[n_in - 1]) * in.geo->stride + s0;
+#pragma unroll
+            for (int k = 0; k < K; ++k) {
+                v[k] = load_pair(p + 2u * kLanes * k);
+                if (n_in - 1 >= n_pos) { v[k].x = ~v[k].x; v[k].y = ~v[k].y; }
+            }
+        }
+    } else if (type == 0u || type == 5u) {
+#pragma unroll
+        for (int k = 0; k < K; ++k) v[k] = {~0ull, ~0ull};
+        fold_simple<K, 0, false>(v, ca, n_in, n_pos, in, s0);
+    } else if (type == 3u) {
+#pragma unroll
+        for (int k = 0; k < K; ++k) v[k] = {0, 0};
+        fold_simple<K, 2, false>(v, ca, n_in, n_pos, in, s0);
+    } else {
+#pragma unroll
+        for (int k = 0; k < K; ++k) v[k] = {0, 0};
+        fold_simple<K, 1, false>(v, ca, n_in, n_pos, in, s0);
+    }
+    const bool invert = type == 4u || type == 5u || type == 6u;  // NOT, NAND, NOR
+    uint32_t pc = 0;
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        if (invert) { v[k].x = ~v[k].x; v[k].y = ~v[k].y; }
+        pc += live_popcount(v[k].x, v[k].y, s0 + 2u * kLanes * k, in.geo->n_bp);
+    }
+    if (c.type_k_flags & kGateTallied) tally_wave(in.ones, c.out_row, pc);
+}
+
+// AND / OR / XOR families over K pairs per lane (lane words s0 + 512 k):
+// row inputs two at a time so 2K 16-byte loads are in flight per lane.
+template <int K, int Op, bool kFused>  // Op: 0 AND, 1 OR, 2 XOR
+__device__ __forceinline__ void fold_simple(Pair (&r)[K], const uint32_t *__restrict__ a,
+                                            uint32_t n_in, uint32_t n_pos, const Inputs &in,
                                             uint64_t s0) {
     auto apply = [&](Pair &acc, Pair v) {
         if (Op == 0) { acc.x &= v.x; acc.y &= v.y; }
         if (Op == 1) { acc.x |= v.x; acc.y |= v.y; }
         if (Op == 2) { acc.x ^= v.x; acc.y ^= v.y; }
     };
+    const uint64_t stride = in.geo->stride;
     uint32_t i = 0;
-    for (; i + 2 <= n_in; i += 2) {
-        const uint64_t *p0 = words + uint64_t(a[i]) * stride + s0;
-        const uint64_t *p1 = words + uint64_t(a[i + 1]) * stride + s0;
-        Pair v0[K], v1[K];
+    while (i < n_in) {
+        if (kFused && (a[i] & kInlineArg)) {
+            Pair v[K];
+            eval_fused<K>(a[i], in, s0, v);
+            const bool n0 = i >= n_pos;
 #pragma unroll
-        for (int k = 0; k < K; ++k) {
-            v0[k] = load_pair(p0 + 2u * kLanes * k);
-            v1[k] = load_pair(p1 + 2u * kLanes * k);
+            for (int k = 0; k < K; ++k) {
+                if (n0) { v[k].x = ~v[k].x; v[k].y = ~v[k].y; }
+                apply(r[k], v[k]);
+            }
+            i += 1;
+            continue;
         }
-        const bool n0 = i >= n_pos, n1 = i + 1 >= n_pos;
+        if (i + 1 < n_in && !(kFused && (a[i + 1] & kInlineArg))) {
+            const uint64_t *p0 = in.words + uint64_t(a[i]) * stride + s0;
+            const uint64_t *p1 = in.words + uint64_t(a[i + 1]) * stride + s0;
+            Pair v0[K], v1[K];
 #pragma unroll
-        for (int k = 0; k < K; ++k) {
-            if (n0) { v0[k].x = ~v0[k].x; v0[k].y = ~v0[k].y; }
-            if (n1) { v1[k].x = ~v1[k].x; v1[k].y = ~v1[k].y; }
-            apply(r[k], v0[k]);
-            apply(r[k], v1[k]);
+            for (int k = 0; k < K; ++k) {
+                v0[k] = load_pair(p0 + 2u * kLanes * k);
+                v1[k] = load_pair(p1 + 2u * kLanes * k);
+            }
+            const bool n0 = i >= n_pos, n1 = i + 1 >= n_pos;
+#pragma unroll
+            for (int k = 0; k < K; ++k) {
+                if (n0) { v0[k].x = ~v0[k].x; v0[k].y = ~v0[k].y; }
+                if (n1) { v1[k].x = ~v1[k].x; v1[k].y = ~v1[k].y; }
+                apply(r[k], v0[k]);
+                apply(r[k], v1[k]);
+            }
+            i += 2;
+            continue;
         }
-    }
-    if (i < n_in) {
-        const uint64_t *p0 = words + uint64_t(a[i]) * stride + s0;
+        const uint64_t *p0 = in.words + uint64_t(a[i]) * stride + s0;
         const bool n0 = i >= n_pos;
 #pragma unroll
         for (int k = 0; k < K; ++k) {
@@ -111,18 +203,26 @@ __device__ __forceinline__ void fold_simple(Pair (&r)[K], const uint32_t *__rest
             if (n0) { v.x = ~v.x; v.y = ~v.y; }
             apply(r[k], v);
         }
+        i += 1;
     }
 }
 
 // Up to 15 inputs: four planes hold the count exactly; k8 above 15 can never
 // be met (the comparison sees a k bit above the planes as unmet).
+template <bool kFused>
 __device__ __forceinline__ Pair atleast_pair4(const uint32_t *__restrict__ a, uint32_t n_in,
-                                              uint32_t n_pos, uint32_t k8,
-                                              const uint64_t *__restrict__ words, uint64_t stride,
+                                              uint32_t n_pos, uint32_t k8, const Inputs &in,
                                               uint64_t s) {
     uint64_t cx[4] = {0, 0, 0, 0}, cy[4] = {0, 0, 0, 0};
     for (uint32_t i = 0; i < n_in; ++i) {
-        Pair v = load_pair(words + uint64_t(a[i]) * stride + s);
+        Pair v;
+        if (kFused && (a[i] & kInlineArg)) {
+            Pair f[1];
+            eval_fused<1>(a[i], in, s, f);
+            v = f[0];
+        } else {
+            v = load_pair(in.words + uint64_t(a[i]) * in.geo->stride + s);
+        }
         if (i >= n_pos) { v.x = ~v.x; v.y = ~v.y; }
 #pragma unroll
         for (int b = 0; b < 4; ++b) {
@@ -137,15 +237,15 @@ __device__ __forceinline__ Pair atleast_pair4(const uint32_t *__restrict__ a, ui
     return {atleast_compare<uint64_t, 4>(cx, k8), atleast_compare<uint64_t, 4>(cy, k8)};
 }
 
-// Any fan-in: eight planes hold the reference's 8-bit count, which wraps at
-// 256 as its uint8 counter does; one word at a time to bound registers.
+// Any fan-in (row inputs only): eight planes hold the reference's 8-bit
+// count, which wraps at 256 as its uint8 counter does; one word at a time to
+// bound registers.
 __device__ __forceinline__ uint64_t atleast_word8(const uint32_t *__restrict__ a, uint32_t n_in,
-                                                  uint32_t n_pos, uint32_t k8,
-                                                  const uint64_t *__restrict__ words,
-                                                  uint64_t stride, uint64_t s) {
+                                                  uint32_t n_pos, uint32_t k8, const Inputs &in,
+                                                  uint64_t s) {
     uint64_t c[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     for (uint32_t i = 0; i < n_in; ++i) {
-        uint64_t v = words[uint64_t(a[i]) * stride + s];
+        uint64_t v = in.words[uint64_t(a[i]) * in.geo->stride + s];
         if (i >= n_pos) v = ~v;
 #pragma unroll
         for (int b = 0; b < 8; ++b) {
@@ -157,35 +257,31 @@ __device__ __forceinline__ uint64_t atleast_word8(const uint32_t *__restrict__ a
     return atleast_compare<uint64_t, 8>(c, k8);
 }
 
-__device__ __forceinline__ Pair atleast_pair(const uint32_t *__restrict__ a, uint32_t n_in,
-                                             uint32_t n_pos, uint32_t k8,
-                                             const uint64_t *__restrict__ words, uint64_t stride,
-                                             uint64_t s) {
-    if (n_in <= 15u) return atleast_pair4(a, n_in, n_pos, k8, words, stride, s);
-    return {atleast_word8(a, n_in, n_pos, k8, words, stride, s),
-            atleast_word8(a, n_in, n_pos, k8, words, stride, s + 1)};
-}
-
-template <int K>
-__device__ __forceinline__ void gate_task_k(const GateDesc &d, const uint32_t *__restrict__ a,
-                                            uint64_t *__restrict__ words,
-                                            unsigned long long *__restrict__ ones,
-                                            const LaunchGeometry &geo, uint64_t base) {
+// The gate's words for pairs k = 0..K-1 (no store, no tally of its own).
+// Fused gates appear only under AND/OR/XOR/NAND/NOR and ATLEAST with at most
+// 15 inputs, never under NOT/NULL or a wide ATLEAST.
+template <int K, bool kFused>
+__device__ __forceinline__ void eval_gate(const GateDesc &d, const uint32_t *__restrict__ a,
+                                          const Inputs &in, uint64_t s0, Pair (&r)[K]) {
     const uint32_t n_in = d.counts & 0xFFFFu;
     const uint32_t n_pos = d.counts >> 16;
     const uint32_t type = d.type_k_flags & 0xFFu;
     const uint32_t k8 = (d.type_k_flags >> 8) & 0xFFu;
-    const uint64_t s0 = base + threadIdx.x * 2u;
-    Pair r[K];
     if (type == 2u) {  // ATLEAST
 #pragma unroll
-        for (int k = 0; k < K; ++k)
-            r[k] = atleast_pair(a, n_in, n_pos, k8, words, geo.stride, s0 + 2u * kLanes * k);
+        for (int k = 0; k < K; ++k) {
+            const uint64_t s = s0 + 2u * kLanes * k;
+            if (n_in <= 15u)
+                r[k] = atleast_pair4<kFused>(a, n_in, n_pos, k8, in, s);
+            else
+                r[k] = {atleast_word8(a, n_in, n_pos, k8, in, s),
+                        atleast_word8(a, n_in, n_pos, k8, in, s + 1)};
+        }
     } else if (type == 4u || type == 7u) {  // NOT, NULL: the last input
 #pragma unroll
         for (int k = 0; k < K; ++k) r[k] = {0, 0};
         if (n_in) {
-            const uint64_t *p = words + uint64_t(a[n_in - 1]) * geo.stride + s0;
+            const uint64_t *p = in.words + uint64_t(a[n_in - 1]) * in.geo->stride + s0;
 #pragma unroll
             for (int k = 0; k < K; ++k) {
                 r[k] = load_pair(p + 2u * kLanes * k);
@@ -197,27 +293,37 @@ __device__ __forceinline__ void gate_task_k(const GateDesc &d, const uint32_t *_
 #pragma unroll
         for (int k = 0; k < K; ++k) r[k] = and_like ? Pair{~0ull, ~0ull} : Pair{0, 0};
         if (and_like)
-            fold_simple<K, 0>(r, a, n_in, n_pos, words, geo.stride, s0);
+            fold_simple<K, 0, kFused>(r, a, n_in, n_pos, in, s0);
         else if (type == 3u)
-            fold_simple<K, 2>(r, a, n_in, n_pos, words, geo.stride, s0);
+            fold_simple<K, 2, kFused>(r, a, n_in, n_pos, in, s0);
         else
-            fold_simple<K, 1>(r, a, n_in, n_pos, words, geo.stride, s0);
+            fold_simple<K, 1, kFused>(r, a, n_in, n_pos, in, s0);
     }
     if (type == 4u || type == 5u || type == 6u) {  // NOT, NAND, NOR
 #pragma unroll
         for (int k = 0; k < K; ++k) { r[k].x = ~r[k].x; r[k].y = ~r[k].y; }
     }
+}
+
+template <int K>
+__device__ __forceinline__ void gate_task_k(const GateDesc &d, const uint32_t *__restrict__ a,
+                                            const Inputs &in, uint64_t *__restrict__ words,
+                                            uint64_t base) {
+    const uint64_t s0 = base + threadIdx.x * 2u;
+    Pair r[K];
+    eval_gate<K, true>(d, a, in, s0, r);
     if (!(d.type_k_flags & kGateNoStore)) {
-        uint64_t *out = words + uint64_t(d.out_row) * geo.stride + s0;
+        uint64_t *out = words + uint64_t(d.out_row) * in.geo->stride + s0;
 #pragma unroll
         for (int k = 0; k < K; ++k)
-            *reinterpret_cast<ulonglong2 *>(out + 2u * kLanes * k) = make_ulonglong2(r[k].x, r[k].y);
+            store_pair(out + 2u * kLanes * k, r[k].x, r[k].y);
     }
     if (d.type_k_flags & kGateTallied) {
         uint32_t pc = 0;
 #pragma unroll
-        for (int k = 0; k < K; ++k) pc += live_popcount(r[k].x, r[k].y, s0 + 2u * kLanes * k, geo.n_bp);
-        tally_wave(ones, d.out_row, pc);
+        for (int k = 0; k < K; ++k)
+            pc += live_popcount(r[k].x, r[k].y, s0 + 2u * kLanes * k, in.geo->n_bp);
+        tally_wave(in.ones, d.out_row, pc);
     }
 }
 
@@ -230,10 +336,11 @@ __device__ __forceinline__ void gate_task(const PassLaunch &L, uint64_t *__restr
     const GateDesc d = L.gates[gi];
     const uint32_t *a = L.arg_rows + d.arg_begin;
     const uint64_t base = uint64_t(chunk) * geo.chunk_words;
+    const Inputs in{&L, words, ones, &geo};
     switch (geo.chunk_words) {
-    case 2048: gate_task_k<4>(d, a, words, ones, geo, base); break;
-    case 1024: gate_task_k<2>(d, a, words, ones, geo, base); break;
-    default: gate_task_k<1>(d, a, words, ones, geo, base); break;
+    case 2048: gate_task_k<4>(d, a, in, words, base); break;
+    case 1024: gate_task_k<2>(d, a, in, words, base); break;
+    default: gate_task_k<1>(d, a, in, words, base); break;
     }
 }
 

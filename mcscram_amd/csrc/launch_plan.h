@@ -18,7 +18,10 @@ struct RowMap {
 };
 
 struct TreeLaunchPlan {
-    std::vector<GateDesc> gates;         // all launches, in order
+    // Task descriptors of all launches in order, then the descriptors of
+    // fused gates (from launch_begin.back()), which tasks reference by
+    // kInlineArg | index among their arguments.
+    std::vector<GateDesc> gates;
     std::vector<uint32_t> args[2];       // argument rows with event buffer 0 / 1
     std::vector<uint32_t> launch_begin;  // first gate of each launch (size L + 1)
     std::vector<double> launch_cost;     // words moved per word of pass (size L)

@@ -263,4 +263,74 @@ Schedule build_schedule(const Plan &plan, uint32_t max_fan_in) {
     return sch;
 }
 
+
+void fuse_single_consumer(Schedule &sch, const Plan &plan, uint32_t max_rows) {
+    const int32_t n_nodes = plan.n_nodes();
+    const size_t n_ids = size_t(n_nodes) + size_t(sch.n_aux);
+    std::vector<int32_t> n_cons(n_ids, 0), consumer(n_ids, -1);
+    std::vector<ExecGate *> by_out(n_ids, nullptr);
+    std::vector<uint32_t> rows(n_ids, 0);
+    for (auto &layer : sch.layers)
+        for (ExecGate &g : layer) {
+            by_out[size_t(g.out)] = &g;
+            rows[size_t(g.out)] = uint32_t(g.pos_in.size() + g.neg_in.size());
+            for (const auto *v : {&g.pos_in, &g.neg_in})
+                for (int32_t i : *v) {
+                    ++n_cons[size_t(i)];
+                    consumer[size_t(i)] = g.out;
+                }
+        }
+    std::vector<uint8_t> removed(n_ids, 0), has_child(n_ids, 0);
+    for (auto &layer : sch.layers)
+        for (ExecGate &g : layer) {
+            const int32_t id = g.out;
+            if (id >= n_nodes || id < plan.n_events() || id == plan.root_node) continue;
+            if (n_cons[size_t(id)] != 1 || has_child[size_t(id)] || g.type == MCS_ATLEAST) continue;
+            ExecGate *c = by_out[size_t(consumer[size_t(id)])];
+            if (!c) continue;
+            const uint32_t c_in = uint32_t(c->pos_in.size() + c->neg_in.size());
+            const bool bitwise = c->type == MCS_AND || c->type == MCS_OR || c->type == MCS_XOR ||
+                                 c->type == MCS_NAND || c->type == MCS_NOR;
+            const bool small_atleast = c->type == MCS_ATLEAST && c_in <= 15;
+            if (!bitwise && !small_atleast) continue;
+            if (rows[size_t(c->out)] - 1 + rows[size_t(id)] > max_rows) continue;
+            c->inl.push_back(g);
+            rows[size_t(c->out)] += rows[size_t(id)] - 1;
+            has_child[size_t(c->out)] = 1;
+            removed[size_t(id)] = 1;
+        }
+    // Rebuild the layers: a task's depth is one more than the deepest row it
+    // reads, its fused gates' inputs included.
+    std::vector<ExecGate> all;
+    for (auto &layer : sch.layers)
+        for (ExecGate &g : layer)
+            if (!removed[size_t(g.out)]) all.push_back(std::move(g));
+    std::vector<int32_t> depth(n_ids, 0);
+    std::vector<std::vector<ExecGate>> layers;
+    for (ExecGate &g : all) {
+        int32_t d = 0;
+        auto row_depth = [&](int32_t i) { d = std::max(d, depth[size_t(i)]); };
+        for (const auto *v : {&g.pos_in, &g.neg_in})
+            for (int32_t i : *v) {
+                const ExecGate *ch = nullptr;
+                for (const ExecGate &c : g.inl)
+                    if (c.out == i) ch = &c;
+                if (!ch) {
+                    row_depth(i);
+                    continue;
+                }
+                for (const auto *w : {&ch->pos_in, &ch->neg_in})
+                    for (int32_t j : *w) row_depth(j);
+            }
+        depth[size_t(g.out)] = d + 1;
+        for (const ExecGate &c : g.inl) depth[size_t(c.out)] = d + 1;
+        if (layers.size() < size_t(d) + 1) layers.resize(size_t(d) + 1);
+        layers[size_t(d)].push_back(std::move(g));
+    }
+    layers.erase(std::remove_if(layers.begin(), layers.end(),
+                                [](const std::vector<ExecGate> &l) { return l.empty(); }),
+                 layers.end());
+    sch.layers = std::move(layers);
+}
+
 }  // namespace mcs

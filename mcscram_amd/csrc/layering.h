@@ -52,6 +52,10 @@ struct ExecGate {
     uint8_t type;
     int32_t k;
     std::vector<int32_t> pos_in, neg_in;
+    // Gates evaluated inside this gate's task (see fuse_single_consumer):
+    // each appears once among pos_in / neg_in by its node id, and its own
+    // inputs are rows.
+    std::vector<ExecGate> inl;
 };
 
 // Execution schedule: the plan's gates in dependency layers, with every
@@ -68,5 +72,19 @@ struct Schedule {
 };
 
 Schedule build_schedule(const Plan &plan, uint32_t max_fan_in);
+
+// Layer fusion for the layered kernel. A gate of the plan whose word is
+// read by exactly one other gate (its consumer), that is not the root and
+// not an ATLEAST gate (whose bit-sliced counter would crowd the consumer's
+// registers), is evaluated inside its consumer's task, so its word never
+// reaches HBM:
+// it is computed in registers and folded in as one more input, and its tally
+// is counted there. One level: a gate that holds fused gates is not fused
+// itself. Consumers are AND/OR/XOR/NAND/NOR (any order of inputs gives the
+// same word) or ATLEAST with at most 15 inputs; a consumer task reads at most
+// max_rows rows in all. Partial folds of split wide gates keep their own
+// tasks. Afterwards the layers are rebuilt from the fused gates' inputs, so
+// a consumer can move to an earlier launch.
+void fuse_single_consumer(Schedule &sch, const Plan &plan, uint32_t max_rows);
 
 }  // namespace mcs

@@ -27,7 +27,8 @@ using mcs::fail;
 
 namespace {
 
-constexpr uint32_t kUaMaxFanIn = 64;
+constexpr uint32_t kUaSplitFanIn = 64;  // as the single-tree engine
+constexpr uint32_t kUaTaskRows = 64;
 
 uint32_t pick_chunk_words(uint64_t n_bp) {
     if (n_bp >= (uint64_t(1) << 18)) return 2048;
@@ -103,6 +104,7 @@ mcs_status enqueue_run(mcs_ua *u, uint32_t n_passes) {
             for (size_t l = 0; l + 1 < u->lp.launch_begin.size(); ++l) {
                 mcs::PassLaunch pl = base;
                 pl.gates = u->d_gates + u->lp.launch_begin[l];
+                pl.tree_gates = u->d_gates;
                 pl.arg_rows = u->d_args[area];
                 pl.n_gates = u->lp.launch_begin[l + 1] - u->lp.launch_begin[l];
                 if (prefetch) {
@@ -153,7 +155,8 @@ mcs_status mcs_ua_create(const mcs_pdag *pdag, const mcs_sample_shape *shape, in
     std::string err;
     if (!mcs::build_plan(*pdag, u->plan, err)) return fail(MCS_ERR_GRAPH, err);
     const mcs::Plan &plan = u->plan;
-    u->sch = mcs::build_schedule(plan, kUaMaxFanIn);
+    u->sch = mcs::build_schedule(plan, kUaSplitFanIn);
+    mcs::fuse_single_consumer(u->sch, plan, kUaTaskRows);
     u->n_ev = uint32_t(plan.n_events());
     u->gate_rows = uint32_t(plan.n_gates() + u->sch.n_aux);
     u->n_samples = n_samples;

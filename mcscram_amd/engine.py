@@ -51,7 +51,7 @@ class _Info(C.Structure):
                 ("iteration", C.c_uint32), ("num_events", C.c_int32), ("num_gates", C.c_int32),
                 ("num_layers", C.c_int32), ("num_tallies", C.c_int32), ("num_args", C.c_uint64),
                 ("device_bytes", C.c_uint64), ("event_buffers", C.c_int32),
-                ("launches_per_pass", C.c_uint32)]
+                ("launches_per_pass", C.c_uint32), ("fused_gates", C.c_int32)]
 
 
 class _BatchConfig(C.Structure):
