@@ -103,6 +103,47 @@ def test_large_tree_prefix_bit_exact(gpu):
     _check_tallies(pd, 1 << 14, tally=[int(g) for g in pd.gate_index])
 
 
+def test_large_tree_parity_run_bit_exact(gpu):
+    # The parity run of the large workload: one pass of 2^18 words (2^24
+    # trials), every gate and every basic event, against the oracle (about a
+    # minute of CPU on 16 threads).
+    pd = mc.large_tree()
+    _check_tallies(pd, 1 << 18)
+
+
+def test_large_tree_full_size_invariants(gpu):
+    # 2^32 trials (256 passes of 2^24), every node tallied. Properties that
+    # hold at any size, with c_i the count of trials on which input i is true
+    # (total - ones for a complemented input): a basic event's count is
+    # Binomial(2^32, p); AND <= min c_i; max c_i <= OR <= min(sum c_i, total);
+    # k * ATLEAST <= sum c_i (fewer than 256 inputs).
+    pd = mc.large_tree()
+    tally = _all_nodes(pd)
+    with mc.LayerManager(pd, 1 << 18, tally=tally) as eng:
+        eng.pass_wait_collect(256, passes_between_waits=32)
+        ones = _engine_ones(eng)
+        total = eng.collect_tallies()[0].total_bits
+    assert total == 1 << 32
+    n = float(total)
+    for v, p in zip(pd.var_index, pd.var_p):
+        mu, sd = n * p, np.sqrt(n * p * (1 - p))
+        assert abs(ones[int(v)] - mu) <= 5.5 * sd + 1, (int(v), ones[int(v)], mu)
+    checked = 0
+    for s, g in enumerate(pd.gate_index):
+        c = [ones[int(x)] if x > 0 else total - ones[int(-x)] for x in pd.gate_args(s)]
+        t, got = pd.gate_type[s], ones[int(g)]
+        if t == mc.pdag.AND:
+            assert got <= min(c), int(g)
+        elif t == mc.pdag.OR:
+            assert max(c) <= got <= min(sum(c), total), int(g)
+        elif t == mc.pdag.ATLEAST:
+            assert int(pd.gate_k[s]) * got <= sum(c), int(g)
+        else:
+            continue
+        checked += 1
+    assert checked == pd.n_gates
+
+
 @pytest.mark.parametrize("name", sorted(core_models()))
 def test_core_known_answers_on_gpu(gpu, name):
     pd, exact = core_models()[name]
