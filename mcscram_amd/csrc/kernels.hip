@@ -81,147 +81,211 @@ struct Pair {
     uint64_t x, y;
 };
 
-__device__ __forceinline__ Pair load_pair(const uint64_t *__restrict__ p) {
-    const u64x2 v = __builtin_nontemporal_load(reinterpret_cast<const u64x2 *>(p));
+// A gate task's view of the words matrix. Each row input is read through a
+// buffer resource built on the scalar unit for the task's chunk of that row
+// (row index read once into an SGPR), with the lane's byte offsets for the K
+// pairs held in VGPRs for the whole task, so an input costs no vector
+// address arithmetic. Lane l holds words base + 2 l + 512 k, k = 0..K-1, at
+// byte offset 16 l + 4096 k of the chunk; the resource's size is the chunk's,
+// so no access can leave it.
+struct Task {
+    const PassLaunch *L;
+    uint64_t *__restrict__ words;
+    unsigned long long *__restrict__ ones;
+    uint64_t stride;      // words per row
+    uint64_t base;        // first word of the chunk
+    uint64_t n_bp;        // live words per row
+    uint32_t lane_off;    // 16 * lane
+    uint32_t chunk_bytes; // 8 * chunk words
+};
+
+constexpr int32_t kBufferWord3 = 0x00020000;  // raw buffer, 32-bit format fields (gfx9)
+constexpr int32_t kNonTemporal = 2;           // cache policy: no temporal reuse expected
+
+__device__ __forceinline__ uint32_t uniform(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
+
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t chunk_of(const Task &t, uint32_t row) {
+    return __builtin_amdgcn_make_buffer_rsrc(t.words + (uint64_t(row) * t.stride + t.base), 0,
+                                             int32_t(t.chunk_bytes), kBufferWord3);
+}
+
+__device__ __forceinline__ Pair load_k(__amdgpu_buffer_rsrc_t r, const Task &t, int k) {
+    const u64x2 v = __builtin_bit_cast(
+        u64x2, __builtin_amdgcn_raw_buffer_load_b128(r, int32_t(t.lane_off + 4096u * uint32_t(k)),
+                                                     0, kNonTemporal));
     return {v.x, v.y};
 }
 
-// Inputs of a task. A row input is a 16-byte load per pair; with kFused an
-// argument may instead reference a fused gate (kInlineArg), whose word the
-// task computes in registers from that gate's own row inputs and counts in
-// that gate's tally. Fused gates never reference fused gates.
-struct Inputs {
-    const PassLaunch *L;
-    const uint64_t *__restrict__ words;
-    unsigned long long *__restrict__ ones;
-    const LaunchGeometry *geo;
-};
+__device__ __forceinline__ uint64_t lane_word(const Task &t, int k, int h) {
+    return t.base + 2u * threadIdx.x + 512u * uint32_t(k) + uint32_t(h);
+}
 
-template <int K, bool kFused>
-__device__ __forceinline__ void eval_gate(const GateDesc &d, const uint32_t *__restrict__ a,
-                                          const Inputs &in, uint64_t s0, Pair (&r)[K]);
+// Truth table of v_bitop3_b32 for acc <op> v0' <op> v1', v' = v or ~v; the
+// table is indexed by 4 acc + 2 v0 + v1 (operand order S0, S1, S2). With
+// kTwo false, v1 is ignored (a one-input fold).
+template <int Op, bool kN0, bool kN1, bool kTwo>
+constexpr uint32_t fold_table() {
+    uint32_t t = 0;
+    for (int i = 0; i < 8; ++i) {
+        const bool acc = (i >> 2) & 1;
+        const bool a = (((i >> 1) & 1) != 0) != kN0;
+        const bool b = ((i & 1) != 0) != kN1;
+        bool r;
+        if (Op == 0) r = acc && a && (!kTwo || b);
+        else if (Op == 1) r = acc || a || (kTwo && b);
+        else r = acc != a != (kTwo && b);
+        t |= uint32_t(r) << i;
+    }
+    return t;
+}
+
+template <uint32_t T>
+__device__ __forceinline__ uint64_t bitop3_64(uint64_t a, uint64_t b, uint64_t c) {
+    const uint32_t lo = __builtin_amdgcn_bitop3_b32(uint32_t(a), uint32_t(b), uint32_t(c), T);
+    const uint32_t hi =
+        __builtin_amdgcn_bitop3_b32(uint32_t(a >> 32), uint32_t(b >> 32), uint32_t(c >> 32), T);
+    return uint64_t(hi) << 32 | lo;
+}
+
+// r <op>= v0' <op> v1': one bitop3 per 32 bits for two inputs.
+template <int Op, bool kN0, bool kN1, int K>
+__device__ __forceinline__ void fold2(Pair (&r)[K], const Pair (&v0)[K], const Pair (&v1)[K]) {
+    constexpr uint32_t T = fold_table<Op, kN0, kN1, true>();
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        r[k].x = bitop3_64<T>(r[k].x, v0[k].x, v1[k].x);
+        r[k].y = bitop3_64<T>(r[k].y, v0[k].y, v1[k].y);
+    }
+}
+
+template <int Op, int K>
+__device__ __forceinline__ void fold1(Pair (&r)[K], const Pair (&v)[K], bool neg) {
+    constexpr uint32_t TP = fold_table<Op, false, false, false>();
+    constexpr uint32_t TN = fold_table<Op, true, false, false>();
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        if (neg) {
+            r[k].x = bitop3_64<TN>(r[k].x, v[k].x, v[k].x);
+            r[k].y = bitop3_64<TN>(r[k].y, v[k].y, v[k].y);
+        } else {
+            r[k].x = bitop3_64<TP>(r[k].x, v[k].x, v[k].x);
+            r[k].y = bitop3_64<TP>(r[k].y, v[k].y, v[k].y);
+        }
+    }
+}
 
 template <int K, int Op, bool kFused>
 __device__ __forceinline__ void fold_simple(Pair (&r)[K], const uint32_t *__restrict__ a,
-                                            uint32_t n_in, uint32_t n_pos, const Inputs &in,
-                                            uint64_t s0);
+                                            uint32_t n_in, uint32_t n_pos, const Task &t,
+                                            int k0 = 0);
 
-// Word pairs k = 0..K-1 (lane words s0 + 512 k) of a fused gate, tallied.
-// Fused gates are AND/OR/XOR/NAND/NOR/NOT/NULL over rows.
+// Word pairs k0 .. k0+K-1 of a fused gate (AND/OR/XOR/NAND/NOR/NOT/NULL over
+// rows), tallied: the popcount is added to the gate's counter here.
 template <int K>
-__device__ __forceinline__ void eval_fused(uint32_t ref, const Inputs &in, uint64_t s0,
-                                           Pair (&v)[K]) {
-    const GateDesc c = in.L->tree_gates[ref & ~kInlineArg];
-    const uint32_t *ca = in.L->arg_rows + c.arg_begin;
+__device__ __forceinline__ void eval_fused(uint32_t ref, const Task &t, Pair (&v)[K], int k0 = 0) {
+    const GateDesc c = t.L->tree_gates[ref & ~kInlineArg];
+    const uint32_t *ca = t.L->arg_rows + c.arg_begin;
     const uint32_t n_in = c.counts & 0xFFFFu, n_pos = c.counts >> 16;
     const uint32_t type = c.type_k_flags & 0xFFu;
     if (type == 4u || type == 7u) {  // NOT, NULL: the last input
 #pragma unroll
         for (int k = 0; k < K; ++k) v[k] = {0, 0};
         if (n_in) {
-            const uint64_t *p = in.words + uint64_t(ca[n_in - 1]) * in.geo->stride + s0;
+            const __amdgpu_buffer_rsrc_t rp = chunk_of(t, uniform(ca[n_in - 1]));
 #pragma unroll
             for (int k = 0; k < K; ++k) {
-                v[k] = load_pair(p + 2u * kLanes * k);
+                v[k] = load_k(rp, t, k0 + k);
                 if (n_in - 1 >= n_pos) { v[k].x = ~v[k].x; v[k].y = ~v[k].y; }
             }
         }
     } else if (type == 0u || type == 5u) {
 #pragma unroll
         for (int k = 0; k < K; ++k) v[k] = {~0ull, ~0ull};
-        fold_simple<K, 0, false>(v, ca, n_in, n_pos, in, s0);
+        fold_simple<K, 0, false>(v, ca, n_in, n_pos, t, k0);
     } else if (type == 3u) {
 #pragma unroll
         for (int k = 0; k < K; ++k) v[k] = {0, 0};
-        fold_simple<K, 2, false>(v, ca, n_in, n_pos, in, s0);
+        fold_simple<K, 2, false>(v, ca, n_in, n_pos, t, k0);
     } else {
 #pragma unroll
         for (int k = 0; k < K; ++k) v[k] = {0, 0};
-        fold_simple<K, 1, false>(v, ca, n_in, n_pos, in, s0);
+        fold_simple<K, 1, false>(v, ca, n_in, n_pos, t, k0);
     }
     const bool invert = type == 4u || type == 5u || type == 6u;  // NOT, NAND, NOR
     uint32_t pc = 0;
 #pragma unroll
     for (int k = 0; k < K; ++k) {
         if (invert) { v[k].x = ~v[k].x; v[k].y = ~v[k].y; }
-        pc += live_popcount(v[k].x, v[k].y, s0 + 2u * kLanes * k, in.geo->n_bp);
+        pc += live_popcount(v[k].x, v[k].y, lane_word(t, k0 + k, 0), t.n_bp);
     }
-    if (c.type_k_flags & kGateTallied) tally_wave(in.ones, c.out_row, pc);
+    if (c.type_k_flags & kGateTallied) tally_wave(t.ones, c.out_row, pc);
 }
 
-// AND / OR / XOR families over K pairs per lane (lane words s0 + 512 k):
-// row inputs two at a time so 2K 16-byte loads are in flight per lane.
-template <int K, int Op, bool kFused>  // Op: 0 AND, 1 OR, 2 XOR
+// AND / OR / XOR families over K pairs per lane, row inputs two at a time
+// (2K 16-byte loads in flight per lane, one bitop3 per 32 bits for both).
+// With kFused an argument may reference a fused gate (kInlineArg), whose
+// word the task computes in registers; fused gates never reference fused
+// gates.
+template <int K, int Op, bool kFused>
 __device__ __forceinline__ void fold_simple(Pair (&r)[K], const uint32_t *__restrict__ a,
-                                            uint32_t n_in, uint32_t n_pos, const Inputs &in,
-                                            uint64_t s0) {
-    auto apply = [&](Pair &acc, Pair v) {
-        if (Op == 0) { acc.x &= v.x; acc.y &= v.y; }
-        if (Op == 1) { acc.x |= v.x; acc.y |= v.y; }
-        if (Op == 2) { acc.x ^= v.x; acc.y ^= v.y; }
-    };
-    const uint64_t stride = in.geo->stride;
+                                            uint32_t n_in, uint32_t n_pos, const Task &t,
+                                            int k0) {
     uint32_t i = 0;
     while (i < n_in) {
-        if (kFused && (a[i] & kInlineArg)) {
+        const uint32_t ai = uniform(a[i]);
+        if (kFused && (ai & kInlineArg)) {
             Pair v[K];
-            eval_fused<K>(a[i], in, s0, v);
-            const bool n0 = i >= n_pos;
-#pragma unroll
-            for (int k = 0; k < K; ++k) {
-                if (n0) { v[k].x = ~v[k].x; v[k].y = ~v[k].y; }
-                apply(r[k], v[k]);
-            }
+            eval_fused<K>(ai, t, v);
+            fold1<Op, K>(r, v, i >= n_pos);
             i += 1;
             continue;
         }
-        if (i + 1 < n_in && !(kFused && (a[i + 1] & kInlineArg))) {
-            const uint64_t *p0 = in.words + uint64_t(a[i]) * stride + s0;
-            const uint64_t *p1 = in.words + uint64_t(a[i + 1]) * stride + s0;
+        // Two rows at a time, except inside a fused gate, whose loads run
+        // while the consumer's accumulator is live (register pressure).
+        const uint32_t aj = i + 1 < n_in ? uniform(a[i + 1]) : kInlineArg;
+        if (kFused && i + 1 < n_in && !(aj & kInlineArg)) {
+            const __amdgpu_buffer_rsrc_t p0 = chunk_of(t, ai), p1 = chunk_of(t, aj);
             Pair v0[K], v1[K];
 #pragma unroll
             for (int k = 0; k < K; ++k) {
-                v0[k] = load_pair(p0 + 2u * kLanes * k);
-                v1[k] = load_pair(p1 + 2u * kLanes * k);
+                v0[k] = load_k(p0, t, k0 + k);
+                v1[k] = load_k(p1, t, k0 + k);
             }
             const bool n0 = i >= n_pos, n1 = i + 1 >= n_pos;
-#pragma unroll
-            for (int k = 0; k < K; ++k) {
-                if (n0) { v0[k].x = ~v0[k].x; v0[k].y = ~v0[k].y; }
-                if (n1) { v1[k].x = ~v1[k].x; v1[k].y = ~v1[k].y; }
-                apply(r[k], v0[k]);
-                apply(r[k], v1[k]);
-            }
+            if (!n1)
+                fold2<Op, false, false>(r, v0, v1);
+            else if (n0)
+                fold2<Op, true, true>(r, v0, v1);
+            else
+                fold2<Op, false, true>(r, v0, v1);
             i += 2;
             continue;
         }
-        const uint64_t *p0 = in.words + uint64_t(a[i]) * stride + s0;
-        const bool n0 = i >= n_pos;
+        const __amdgpu_buffer_rsrc_t p0 = chunk_of(t, ai);
+        Pair v[K];
 #pragma unroll
-        for (int k = 0; k < K; ++k) {
-            Pair v = load_pair(p0 + 2u * kLanes * k);
-            if (n0) { v.x = ~v.x; v.y = ~v.y; }
-            apply(r[k], v);
-        }
+        for (int k = 0; k < K; ++k) v[k] = load_k(p0, t, k0 + k);
+        fold1<Op, K>(r, v, i >= n_pos);
         i += 1;
     }
 }
 
-// Up to 15 inputs: four planes hold the count exactly; k8 above 15 can never
-// be met (the comparison sees a k bit above the planes as unmet).
+// Up to 15 inputs, pair k: four carry-save planes hold the count exactly; k8
+// above 15 can never be met (the comparison sees a k bit above the planes as
+// unmet).
 template <bool kFused>
 __device__ __forceinline__ Pair atleast_pair4(const uint32_t *__restrict__ a, uint32_t n_in,
-                                              uint32_t n_pos, uint32_t k8, const Inputs &in,
-                                              uint64_t s) {
+                                              uint32_t n_pos, uint32_t k8, const Task &t, int k) {
     uint64_t cx[4] = {0, 0, 0, 0}, cy[4] = {0, 0, 0, 0};
     for (uint32_t i = 0; i < n_in; ++i) {
+        const uint32_t ai = uniform(a[i]);
         Pair v;
-        if (kFused && (a[i] & kInlineArg)) {
+        if (kFused && (ai & kInlineArg)) {
             Pair f[1];
-            eval_fused<1>(a[i], in, s, f);
+            eval_fused<1>(ai, t, f, k);  // pair k only; tallied pair by pair
             v = f[0];
         } else {
-            v = load_pair(in.words + uint64_t(a[i]) * in.geo->stride + s);
+            v = load_k(chunk_of(t, ai), t, k);
         }
         if (i >= n_pos) { v.x = ~v.x; v.y = ~v.y; }
 #pragma unroll
@@ -237,21 +301,24 @@ __device__ __forceinline__ Pair atleast_pair4(const uint32_t *__restrict__ a, ui
     return {atleast_compare<uint64_t, 4>(cx, k8), atleast_compare<uint64_t, 4>(cy, k8)};
 }
 
-// Any fan-in (row inputs only): eight planes hold the reference's 8-bit
-// count, which wraps at 256 as its uint8 counter does; one word at a time to
-// bound registers.
+// Any fan-in (row inputs only), word h of pair k: eight planes hold the
+// reference's 8-bit count, which wraps at 256 as its uint8 counter does.
 __device__ __forceinline__ uint64_t atleast_word8(const uint32_t *__restrict__ a, uint32_t n_in,
-                                                  uint32_t n_pos, uint32_t k8, const Inputs &in,
-                                                  uint64_t s) {
+                                                  uint32_t n_pos, uint32_t k8, const Task &t, int k,
+                                                  int h) {
     uint64_t c[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     for (uint32_t i = 0; i < n_in; ++i) {
-        uint64_t v = in.words[uint64_t(a[i]) * in.geo->stride + s];
+        uint64_t v = __builtin_bit_cast(
+            uint64_t, __builtin_amdgcn_raw_buffer_load_b64(
+                          chunk_of(t, uniform(a[i])),
+                          int32_t(t.lane_off + 4096u * uint32_t(k) + 8u * uint32_t(h)), 0,
+                          kNonTemporal));
         if (i >= n_pos) v = ~v;
 #pragma unroll
         for (int b = 0; b < 8; ++b) {
-            const uint64_t t = c[b] & v;
+            const uint64_t x = c[b] & v;
             c[b] ^= v;
-            v = t;
+            v = x;
         }
     }
     return atleast_compare<uint64_t, 8>(c, k8);
@@ -262,7 +329,7 @@ __device__ __forceinline__ uint64_t atleast_word8(const uint32_t *__restrict__ a
 // 15 inputs, never under NOT/NULL or a wide ATLEAST.
 template <int K, bool kFused>
 __device__ __forceinline__ void eval_gate(const GateDesc &d, const uint32_t *__restrict__ a,
-                                          const Inputs &in, uint64_t s0, Pair (&r)[K]) {
+                                          const Task &t, Pair (&r)[K]) {
     const uint32_t n_in = d.counts & 0xFFFFu;
     const uint32_t n_pos = d.counts >> 16;
     const uint32_t type = d.type_k_flags & 0xFFu;
@@ -270,21 +337,20 @@ __device__ __forceinline__ void eval_gate(const GateDesc &d, const uint32_t *__r
     if (type == 2u) {  // ATLEAST
 #pragma unroll
         for (int k = 0; k < K; ++k) {
-            const uint64_t s = s0 + 2u * kLanes * k;
             if (n_in <= 15u)
-                r[k] = atleast_pair4<kFused>(a, n_in, n_pos, k8, in, s);
+                r[k] = atleast_pair4<kFused>(a, n_in, n_pos, k8, t, k);
             else
-                r[k] = {atleast_word8(a, n_in, n_pos, k8, in, s),
-                        atleast_word8(a, n_in, n_pos, k8, in, s + 1)};
+                r[k] = {atleast_word8(a, n_in, n_pos, k8, t, k, 0),
+                        atleast_word8(a, n_in, n_pos, k8, t, k, 1)};
         }
     } else if (type == 4u || type == 7u) {  // NOT, NULL: the last input
 #pragma unroll
         for (int k = 0; k < K; ++k) r[k] = {0, 0};
         if (n_in) {
-            const uint64_t *p = in.words + uint64_t(a[n_in - 1]) * in.geo->stride + s0;
+            const __amdgpu_buffer_rsrc_t rp = chunk_of(t, uniform(a[n_in - 1]));
 #pragma unroll
             for (int k = 0; k < K; ++k) {
-                r[k] = load_pair(p + 2u * kLanes * k);
+                r[k] = load_k(rp, t, k);
                 if (n_in - 1 >= n_pos) { r[k].x = ~r[k].x; r[k].y = ~r[k].y; }
             }
         }
@@ -293,11 +359,11 @@ __device__ __forceinline__ void eval_gate(const GateDesc &d, const uint32_t *__r
 #pragma unroll
         for (int k = 0; k < K; ++k) r[k] = and_like ? Pair{~0ull, ~0ull} : Pair{0, 0};
         if (and_like)
-            fold_simple<K, 0, kFused>(r, a, n_in, n_pos, in, s0);
+            fold_simple<K, 0, kFused>(r, a, n_in, n_pos, t);
         else if (type == 3u)
-            fold_simple<K, 2, kFused>(r, a, n_in, n_pos, in, s0);
+            fold_simple<K, 2, kFused>(r, a, n_in, n_pos, t);
         else
-            fold_simple<K, 1, kFused>(r, a, n_in, n_pos, in, s0);
+            fold_simple<K, 1, kFused>(r, a, n_in, n_pos, t);
     }
     if (type == 4u || type == 5u || type == 6u) {  // NOT, NAND, NOR
 #pragma unroll
@@ -307,23 +373,26 @@ __device__ __forceinline__ void eval_gate(const GateDesc &d, const uint32_t *__r
 
 template <int K>
 __device__ __forceinline__ void gate_task_k(const GateDesc &d, const uint32_t *__restrict__ a,
-                                            const Inputs &in, uint64_t *__restrict__ words,
-                                            uint64_t base) {
-    const uint64_t s0 = base + threadIdx.x * 2u;
+                                            const Task &t) {
     Pair r[K];
-    eval_gate<K, true>(d, a, in, s0, r);
+    eval_gate<K, true>(d, a, t, r);
     if (!(d.type_k_flags & kGateNoStore)) {
-        uint64_t *out = words + uint64_t(d.out_row) * in.geo->stride + s0;
+        const __amdgpu_buffer_rsrc_t o = chunk_of(t, uniform(d.out_row));
 #pragma unroll
-        for (int k = 0; k < K; ++k)
-            store_pair(out + 2u * kLanes * k, r[k].x, r[k].y);
+        for (int k = 0; k < K; ++k) {
+            u64x2 v;
+            v.x = r[k].x;
+            v.y = r[k].y;
+            __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(__attribute__((ext_vector_type(4))) unsigned, v), o,
+                                                   int32_t(t.lane_off + 4096u * uint32_t(k)), 0,
+                                                   kNonTemporal);
+        }
     }
     if (d.type_k_flags & kGateTallied) {
         uint32_t pc = 0;
 #pragma unroll
-        for (int k = 0; k < K; ++k)
-            pc += live_popcount(r[k].x, r[k].y, s0 + 2u * kLanes * k, in.geo->n_bp);
-        tally_wave(in.ones, d.out_row, pc);
+        for (int k = 0; k < K; ++k) pc += live_popcount(r[k].x, r[k].y, lane_word(t, k, 0), t.n_bp);
+        tally_wave(t.ones, d.out_row, pc);
     }
 }
 
@@ -335,12 +404,12 @@ __device__ __forceinline__ void gate_task(const PassLaunch &L, uint64_t *__restr
     const uint32_t gi = uint32_t(task - uint64_t(chunk) * L.n_gates);
     const GateDesc d = L.gates[gi];
     const uint32_t *a = L.arg_rows + d.arg_begin;
-    const uint64_t base = uint64_t(chunk) * geo.chunk_words;
-    const Inputs in{&L, words, ones, &geo};
+    const Task t{&L, words, ones, geo.stride, uint64_t(chunk) * geo.chunk_words, geo.n_bp,
+                 16u * threadIdx.x, 8u * geo.chunk_words};
     switch (geo.chunk_words) {
-    case 2048: gate_task_k<4>(d, a, in, words, base); break;
-    case 1024: gate_task_k<2>(d, a, in, words, base); break;
-    default: gate_task_k<1>(d, a, in, words, base); break;
+    case 2048: gate_task_k<4>(d, a, t); break;
+    case 1024: gate_task_k<2>(d, a, t); break;
+    default: gate_task_k<1>(d, a, t); break;
     }
 }
 
