@@ -247,6 +247,9 @@ def main() -> int:
     n_bp = args.words
     eng = mc.LayerManager(pd, n_bp, tally=tally)
     info = eng.info
+    # The first pass also runs the first pass's sampler alone (one launch):
+    # its extra time is the sampler's VALU-bound cost, the pass's floor.
+    first = eng.time_passes(1)
     if args.warmup:
         eng.time_passes(args.warmup)
     eng.wait()
@@ -258,6 +261,7 @@ def main() -> int:
 
     trials = args.steps * n_bp * 64
     gate_evals = pd.n_gates * trials
+    sampler_ms = first["kernel_ms"] - dev["kernel_ms"] / args.steps
     value = gate_evals / wall
     ab = algorithmic_bytes(pd, n_bp)
     # pass_layer is the only kernel: every launch carries one gate layer and
@@ -294,6 +298,10 @@ def main() -> int:
         "pass_roofline": {"bound": "hbm", "algorithmic_bytes_per_pass": ab["pass"],
                           "achieved": pass_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                           "frac": pass_gbs / HBM_PEAK_GBS},
+        # The sampler alone issues VALU instructions at the chip's peak rate
+        # (rocprofv3 SQ_INSTS_VALU); a pass cannot be shorter than it.
+        "valu_floor": {"sampler_only_ms": sampler_ms, "pass_ms": dev["kernel_ms"] / args.steps,
+                       "frac": sampler_ms / (dev["kernel_ms"] / args.steps)},
         "device_ms_per_pass": {"kernels": dev["kernel_ms"] / args.steps,
                                "first_to_last": dev["pass_ms"] / args.steps},
         "event_buffers": info["event_buffers"],
