@@ -12,3 +12,4 @@ from .engine import (Batch, LayerManager, Settings, Tally, beta_quantile,  # noq
                      normal_quantile_two_sided, pdag_event_order, pdag_gate_depths,
                      plan_passes, probability_analysis)
 from . import mef  # noqa: F401  (Open-PSA MEF -> PDAG at compilation level 0)
+from . import report  # noqa: F401  (convergence.csv rows)

@@ -31,7 +31,11 @@ STATS = ["model_num_basic_events", "model_num_gates", "pdag_num_gates_total",
          "pdag_num_variables_total", "pdag_num_gates_and", "pdag_num_gates_or",
          "pdag_num_gates_atleast", "pdag_num_gates_xor", "pdag_num_gates_not",
          "pdag_num_gates_nand", "pdag_num_gates_nor", "pdag_num_gates_null",
-         "pdag_num_gates_neg_indices", "pdag_max_gate_fan_in", "pdag_avg_gate_fan_in"]
+         "pdag_num_gates_neg_indices", "pdag_max_gate_fan_in", "pdag_avg_gate_fan_in",
+         "pdag_complement_graph", "pdag_coherent_graph", "pdag_normalized_graph",
+         "pdag_num_gates_pos_and_neg", "pdag_num_variables_neg_indices",
+         "pdag_num_variables_pos_and_neg", "pdag_num_constants", "pdag_has_constants",
+         "pdag_has_null_gates", "pdag_num_substitutions"]
 
 
 def main():

@@ -117,3 +117,24 @@ def test_probability_analysis_entry_on_a_loaded_model(gpu):
     assert passes == (1 << 30) // (WORDS * 64)
     check(tallies[0].num_one_bits, tallies[0].total_bits, 7.13018e-4, 3.2905)
     assert p_total == tallies[0].mean
+
+
+def test_convergence_csv_row_of_a_gpu_run(gpu, tmp_path):
+    """A fixed-N run on baobab2 appends one convergence.csv row whose tally
+    and run columns are the run's own."""
+    from mcscram_amd import report
+    pd = mef.load_pdag(gzip.open(os.path.join(GOLD, "aralia", "baobab2.xml.gz")))
+    path = str(tmp_path / "convergence.csv")
+    s = mc.Settings(num_trials=1 << 28)
+    for _ in range(2):
+        p, tallies, passes = report.probability_analysis_logged(pd, s, WORDS, path,
+                                                                input_file="baobab2.xml")
+    lines = open(path).read().splitlines()
+    assert len(lines) == 3
+    header = lines[0].split(",")
+    row = dict(zip(header, lines[2].split(",")))
+    assert int(row["one_bits"]) == tallies[0].num_one_bits
+    assert int(row["total_bits"]) == 1 << 28 == tallies[0].total_bits
+    assert int(row["shaper_num_iterations"]) == passes == 4
+    assert row["device_type"] == "gpu" and int(row["max_compute_units"]) > 0
+    assert float(row["mean"]) == pytest.approx(p, rel=1e-7)
