@@ -268,7 +268,11 @@ def main() -> int:
     # a slice of the next pass's sampling.
     launch_s = dev["kernel_ms"] / 1e3 / dev["launches"]
     launches_per_pass = dev["launches"] / args.steps
-    bytes_per_launch = (ab["sampler"] + ab["gates"]) / launches_per_pass
+    # Algorithmic bytes of the materialised dataflow, 8 (N_be + N_g + F + T)
+    # per word, tallied words' reads included (the reference's tally kernel
+    # re-reads every tallied node word; here the popcount is fused at
+    # production): the launches of a pass together process one pass of words.
+    bytes_per_launch = ab["pass"] / launches_per_pass
     achieved = bytes_per_launch / launch_s / 1e9
     pass_gbs = ab["pass"] / (wall / args.steps) / 1e9
 
