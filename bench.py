@@ -151,36 +151,55 @@ def bench_fused(args, mc) -> int:
 
 
 def bench_ua(args) -> int:
-    """Surrogate-screened uncertainty analysis of the 2,000-event / 3,000-gate
-    tree (lognormal basic events, median log-uniform in [1e-5, 1e-2], error
-    factor uniform in [3, 10]) against plain batched GPU Monte Carlo over the
-    same held-out parameter samples."""
+    """Surrogate-screened uncertainty analysis against plain batched GPU Monte
+    Carlo over the same held-out parameter samples.
+
+    ua: the 2,000-event / 3,000-gate tree (lognormal basic events, median
+    log-uniform in [1e-5, 1e-2], error factor uniform in [3, 10]); training
+    pairs and outer samples at 2^22 trials, 16,384 outer samples.
+    ua-large: the large tree's gate 64134 (956 events, 210 gates; the tree's
+    own top event is certain), lognormal around the tree's probabilities with
+    error factor uniform in [3, 10]; training pairs at 2^22 trials, 100,000
+    outer samples at 2^24 trials, plain Monte Carlo over all of them."""
     from mcscram_amd import screening as scr
     from mcscram_amd import ua
-    pd, model = ua.uncertainty_tree()
-    words = args.ua_words
+    large = args.workload == "ua-large"
+    if large:
+        pd, model = ua.large_tree_uncertainty()
+        outer = args.ua_outer or 100_000
+        words = args.ua_words or 1 << 18
+        name = (f"surrogate-screened UA, large tree's gate {ua.LARGE_TREE_UA_TOP} "
+                f"({pd.n_vars} events / {pd.n_gates} gates)")
+    else:
+        pd, model = ua.uncertainty_tree()
+        outer = args.ua_outer or 16384
+        words = args.ua_words or 1 << 16
+        name = "surrogate-screened UA, 2,000 events / 3,000 gates"
+    train_words = args.ua_train_words or 1 << 16
     say = lambda m: print(f"[ua] {m}", file=sys.stderr, flush=True)  # noqa: E731
-    say(f"training pairs: {args.ua_train} x {words * 64} trials")
-    ens, info = scr.train_surrogate(pd, model, args.ua_train, words, seed=3,
+    say(f"training pairs: {args.ua_train} x {train_words * 64} trials")
+    ens, info = scr.train_surrogate(pd, model, args.ua_train, train_words, seed=3,
                                     epochs=args.ua_epochs,
                                     progress=lambda d, n: say(f"training pairs {d}/{n}"))
     say(f"trained: {info}")
-    res = scr.screened_uncertainty(pd, model, ens, args.ua_outer, words, seed=1234)
+    res = scr.screened_uncertainty(pd, model, ens, outer, words, seed=1234)
     say(f"screened: {res.seconds}, routed {res.routed.mean():.3f}")
-    plain, t_plain = scr.plain_uncertainty(pd, model, args.ua_outer, words, seed=1234)
+    plain, t_plain = scr.plain_uncertainty(pd, model, outer, words, seed=1234, chunk=8192,
+                                           progress=lambda d, n: say(f"plain Monte Carlo {d}/{n}"))
     say(f"plain Monte Carlo: {t_plain:.1f} s")
     cmp = scr.compare(res, plain)
     t_scr = res.seconds["total"]
     out = {
         "metric": "UA parameter-samples/s at <=2% rel err vs plain MC",
-        "value": args.ua_outer / t_scr, "unit": "samples/s", "n_gpus": 1, "steps": 1,
+        "value": outer / t_scr, "unit": "samples/s", "n_gpus": 1, "steps": 1,
         "warmup": 0, "ms_per_step": t_scr * 1e3, "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "bf16 network, u64 Monte Carlo", "data": "synthetic",
-        "config": {"workload": "surrogate-screened UA, 2,000 events / 3,000 gates",
-                   "training_pairs": args.ua_train, "outer_samples": args.ua_outer,
-                   "trials_per_mc_run": words * 64, "network": "2000-512-512-512-1 GELU x2",
+        "config": {"workload": name, "training_pairs": args.ua_train,
+                   "trials_per_training_pair": train_words * 64, "outer_samples": outer,
+                   "trials_per_mc_run": words * 64,
+                   "network": f"{pd.n_vars}-512-512-512-1 GELU x2",
                    "disagreement": 0.05, "audit": 0.05},
-        "plain_mc_samples_per_s": args.ua_outer / t_plain,
+        "plain_mc_samples_per_s": outer / t_plain,
         "speedup_vs_plain_mc": t_plain / t_scr,
         # The one-off cost of the training pairs and the fit, charged to this
         # one set of outer samples (it is shared by every later analysis of
@@ -216,15 +235,21 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--words", type=int, default=1 << 18, help="uint64 words per pass")
     ap.add_argument("--tree", choices=["large", "small"], default="large")
-    ap.add_argument("--workload", choices=["layered", "batched", "small-fused", "ua"],
+    ap.add_argument("--workload", choices=["layered", "batched", "small-fused", "ua", "ua-large"],
                     default="layered",
                     help="layered: the large tree through the layered engine (default); "
                          "batched: 256 uneven trees, 2^26 trials each, one persistent launch "
                          "per pass; small-fused: the small tree, 2^20 trials, fused kernel; "
-                         "ua: surrogate-screened uncertainty analysis vs plain GPU Monte Carlo")
+                         "ua: surrogate-screened uncertainty analysis vs plain GPU Monte Carlo; "
+                         "ua-large: the same on the large tree's gate 64134, 100,000 outer "
+                         "samples at 2^24 trials")
     ap.add_argument("--ua-train", type=int, default=65536, help="training pairs (ua)")
-    ap.add_argument("--ua-outer", type=int, default=16384, help="outer parameter samples (ua)")
-    ap.add_argument("--ua-words", type=int, default=1 << 16, help="words per Monte Carlo run (ua)")
+    ap.add_argument("--ua-outer", type=int, default=0,
+                    help="outer parameter samples (ua: 16,384; ua-large: 100,000)")
+    ap.add_argument("--ua-words", type=int, default=0,
+                    help="words per outer Monte Carlo run (ua: 2^16; ua-large: 2^18)")
+    ap.add_argument("--ua-train-words", type=int, default=0,
+                    help="words per training Monte Carlo run (2^16)")
     ap.add_argument("--ua-epochs", type=int, default=30)
     ap.add_argument("--trees", type=int, default=256)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
@@ -238,7 +263,7 @@ def main() -> int:
         return 2
 
     import mcscram_amd as mc
-    if args.workload == "ua":
+    if args.workload in ("ua", "ua-large"):
         return bench_ua(args)
     if args.workload != "layered":
         return bench_fused(args, mc)

@@ -68,6 +68,37 @@ class Pdag:
         return Pdag(self.var_index, p, self.gate_index, self.gate_type, self.gate_k,
                     self.arg_off, self.args, self.root, dict(self.meta))
 
+    def subgraph(self, top: int) -> "Pdag":
+        """The PDAG of gate ``top`` alone: the nodes reachable from it, with
+        their indices, probabilities and argument lists unchanged, in their
+        original order, and ``top`` as the root. This is the graph a fault
+        tree whose top event is that gate compiles to."""
+        slot = {int(g): s for s, g in enumerate(self.gate_index)}
+        if int(top) not in slot:
+            raise ValueError("top is not a gate")
+        keep_g = np.zeros(self.n_gates, dtype=bool)
+        keep_v = set()
+        stack = [slot[int(top)]]
+        keep_g[stack[0]] = True
+        while stack:
+            for a in self.gate_args(stack.pop()):
+                a = abs(int(a))
+                s = slot.get(a)
+                if s is None:
+                    keep_v.add(a)
+                elif not keep_g[s]:
+                    keep_g[s] = True
+                    stack.append(s)
+        vmask = np.isin(self.var_index, np.fromiter(keep_v, dtype=np.int64, count=len(keep_v)))
+        gs = np.flatnonzero(keep_g)
+        lens = self.arg_off[gs + 1] - self.arg_off[gs]
+        off = np.zeros(gs.shape[0] + 1, dtype=np.int64)
+        np.cumsum(lens, out=off[1:])
+        args = np.concatenate([self.gate_args(s) for s in gs]).astype(np.int32)
+        return Pdag(self.var_index[vmask].copy(), self.var_p[vmask].copy(), self.gate_index[gs].copy(),
+                    self.gate_type[gs].copy(), self.gate_k[gs].copy(), off, args, int(top),
+                    dict(self.meta, top=int(top)))
+
     def validate(self) -> None:
         idx = np.concatenate([self.var_index, self.gate_index])
         if np.unique(idx).shape[0] != idx.shape[0]:

@@ -139,11 +139,22 @@ def screened_uncertainty(pd: Pdag, model: LognormalModel, ens: Ensemble, n_outer
 
 
 def plain_uncertainty(pd: Pdag, model: LognormalModel, n_outer: int, words: int, seed: int,
-                      id_base: int = 0):
+                      id_base: int = 0, chunk: int = 1 << 30, progress=None):
     """Every outer sample by plain Monte Carlo (same samples, same keys as
-    screened_uncertainty with the same seed). -> (estimates, seconds)."""
+    screened_uncertainty with the same seed), in runs of ``chunk`` samples,
+    each followed by ``progress(done, n_outer)`` if given.
+    -> (estimates, seconds)."""
     p, _ = model.sample(n_outer, seed)
-    return monte_carlo(pd, p, words, (id_base + np.arange(n_outer)).astype(np.uint32))
+    ids = (id_base + np.arange(n_outer)).astype(np.uint32)
+    est = np.empty(n_outer)
+    seconds = 0.0
+    for a in range(0, n_outer, chunk):
+        b = min(n_outer, a + chunk)
+        est[a:b], dt = monte_carlo(pd, p[a:b], words, ids[a:b])
+        seconds += dt
+        if progress:
+            progress(b, n_outer)
+    return est, seconds
 
 
 def compare(screened: ScreeningResult, plain: np.ndarray) -> dict:

@@ -56,6 +56,64 @@ def _oracle_sample(pd, p, n_bp, key, n_iters, tally):
     return orc.run(pd.with_probabilities(p), n_bp, n_iters=n_iters, tally=tally, key=key)[0]
 
 
+def _closure(pd, top):
+    gates = {int(g): i for i, g in enumerate(pd.gate_index)}
+    seen, stack = {top}, [top]
+    while stack:
+        for a in pd.gate_args(gates[stack.pop()]):
+            a = abs(int(a))
+            if a not in seen:
+                seen.add(a)
+                if a in gates:
+                    stack.append(a)
+    return seen
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_subgraph_is_the_gates_own_pdag(seed):
+    pd = mc.random_layered_tree(seed, 40, 50, 5, all_connectives=True, neg_frac=0.2)
+    for top in pd.gate_index[::7]:
+        top = int(top)
+        sub = pd.subgraph(top)
+        sub.validate()
+        assert sub.root == top
+        nodes = _closure(pd, top)
+        assert set(sub.var_index.tolist()) | set(sub.gate_index.tolist()) == nodes
+        # Order, probabilities and argument lists are kept.
+        keep_v = np.isin(pd.var_index, sub.var_index)
+        assert np.array_equal(sub.var_index, pd.var_index[keep_v])
+        assert np.array_equal(sub.var_p, pd.var_p[keep_v])
+        slot = {int(g): i for i, g in enumerate(pd.gate_index)}
+        for i, g in enumerate(sub.gate_index):
+            assert np.array_equal(sub.gate_args(i), pd.gate_args(slot[int(g)]))
+            assert sub.gate_type[i] == pd.gate_type[slot[int(g)]]
+            assert sub.gate_k[i] == pd.gate_k[slot[int(g)]]
+        # The same event: the exact probability of the gate is unchanged.
+        whole = mc.Pdag(pd.var_index, pd.var_p, pd.gate_index, pd.gate_type, pd.gate_k,
+                        pd.arg_off, pd.args, top)
+        p_whole, _ = orc.bdd_probability(whole)
+        p_sub, _ = orc.bdd_probability(sub)
+        assert p_whole == pytest.approx(p_sub, rel=1e-12, abs=1e-300)
+
+
+def test_subgraph_rejects_a_basic_event():
+    pd = mc.small_tree(1)
+    with pytest.raises(ValueError):
+        pd.subgraph(int(pd.var_index[0]))
+
+
+def test_large_tree_uncertainty_model():
+    pd, model = ua.large_tree_uncertainty()
+    pd.validate()
+    assert (pd.n_vars, pd.n_gates, pd.root) == (956, 210, ua.LARGE_TREE_UA_TOP)
+    assert np.all((model.error_factor >= 3.0) & (model.error_factor < 10.0))
+    assert np.array_equal(model.median, pd.var_p)
+    assert np.all((pd.var_p >= 1e-6) & (pd.var_p <= 1e-2))
+    # Neither certain nor negligible at the medians (2^18 trials on the CPU).
+    ones, _ = orc.run(pd, 1 << 12)
+    assert 3e-3 < int(ones[0]) / float(1 << 18) < 1.5e-2
+
+
 @pytest.mark.gpu
 def test_ua_samples_bit_exact_against_oracle(gpu):
     pd = small_random(31, n_be=40, n_gates=60, n_layers=6)
