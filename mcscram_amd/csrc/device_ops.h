@@ -141,10 +141,22 @@ __device__ __forceinline__ uint64_t bernoulli_word(uint32_t c0, uint32_t c1, uin
     return bernoulli_word_t<false>(c0, c1, c2, c3base, k0, k1, thr);
 }
 
+// Sum of v over a fully active wave64, returned as a wave-uniform value.
+// DPP adds inside each 16-lane row (the inputs shifted right by 1, 2 and 3
+// lanes, then the partial sums by 4 lanes into lanes 4-15 and by 8 lanes
+// into lanes 8-15) leave each row's total in its lane 15; row broadcasts of
+// lane 15 into rows 1 and 3, then of lane 31 into rows 2 and 3, leave the
+// wave's total in lane 63. Seven VALU adds and no LDS round trips.
 __device__ __forceinline__ uint32_t wave_sum(uint32_t v) {
-#pragma unroll
-    for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
-    return v;
+    uint32_t s = v;
+    s += uint32_t(__builtin_amdgcn_update_dpp(0, int(v), 0x111, 0xf, 0xf, true));  // row_shr:1
+    s += uint32_t(__builtin_amdgcn_update_dpp(0, int(v), 0x112, 0xf, 0xf, true));  // row_shr:2
+    s += uint32_t(__builtin_amdgcn_update_dpp(0, int(v), 0x113, 0xf, 0xf, true));  // row_shr:3
+    s += uint32_t(__builtin_amdgcn_update_dpp(0, int(s), 0x114, 0xf, 0xe, true));  // row_shr:4
+    s += uint32_t(__builtin_amdgcn_update_dpp(0, int(s), 0x118, 0xf, 0xc, true));  // row_shr:8
+    s += uint32_t(__builtin_amdgcn_update_dpp(0, int(s), 0x142, 0xa, 0xf, true));  // row_bcast:15
+    s += uint32_t(__builtin_amdgcn_update_dpp(0, int(s), 0x143, 0xc, 0xf, true));  // row_bcast:31
+    return uint32_t(__builtin_amdgcn_readlane(int(s), 63));
 }
 
 // ATLEAST: a bit-sliced counter (carry-save planes, plane b = bit b of each

@@ -10,6 +10,8 @@
 // sample pass i + 1 into the other buffer.
 #pragma once
 
+#include <hip/hip_runtime_api.h>
+
 #include <cstdint>
 
 namespace mcs {
@@ -82,11 +84,48 @@ struct PassLaunch {
     uint32_t key0, key1;
 };
 
-}  // namespace mcs
+// Workgroups of a pass_layer launch are dealt to the XCDs round-robin:
+// workgroup b runs on XCD b % kXcds as that XCD's (b / kXcds)-th workgroup.
+// XCD x owns the x-th eighth of the launch's gate tasks (chunk-major, so the
+// gates it evaluates together read the same chunk of their input rows
+// through its own L2) and the x-th eighth of its sampler tasks.
+constexpr uint32_t kXcds = 8;
 
-#include <hip/hip_runtime_api.h>
+struct XcdShare {
+    uint64_t g_lo, s_lo;  // first gate task, first sampler task
+    uint64_t ngx, tx;     // gate tasks, all tasks (gate + sampler)
+};
 
-namespace mcs {
+__host__ __device__ inline XcdShare xcd_share(uint64_t ng, uint64_t ns, uint32_t x) {
+    XcdShare s;
+    s.g_lo = ng * x / kXcds;
+    s.s_lo = ns * x / kXcds;
+    s.ngx = ng * (x + 1) / kXcds - s.g_lo;
+    s.tx = s.ngx + (ns * (x + 1) / kXcds - s.s_lo);
+    return s;
+}
+
+// The XCD interleaves its gate and sampler tasks evenly, without a division
+// on the device: with frac = ceil(ngx * 2^32 / tx) (at most 2^32),
+// gates_before(j) = min(ngx, floor(j * frac / 2^32)) rises by 0 or 1 per
+// workgroup, from 0 at j = 0 to ngx at j = tx. Workgroup j runs gate task
+// gates_before(j) if gates_before(j + 1) > gates_before(j), else sampler
+// task j - gates_before(j); both run through their whole range once.
+// j * frac stays below 2^64 because a launch has fewer than 2^24 workgroups.
+inline uint64_t interleave_frac(uint64_t ngx, uint64_t tx) {
+    if (tx == 0) return 0;
+    const unsigned __int128 n = static_cast<unsigned __int128>(ngx) << 32;
+    return static_cast<uint64_t>((n + tx - 1) / tx);
+}
+
+__host__ __device__ inline uint64_t gates_before(uint64_t j, uint64_t ngx, uint64_t frac) {
+    const uint64_t b = (j * frac) >> 32;
+    return b < ngx ? b : ngx;
+}
+
+struct SplitFracs {
+    uint64_t frac[kXcds];  // interleave_frac of each XCD's share
+};
 
 // Blocks of 256 lanes; the caller checks grid_blocks(...) * 256 < 2^32.
 uint64_t pass_grid_blocks(const PassLaunch &l, const LaunchGeometry &geo);

@@ -29,7 +29,6 @@ namespace mcs {
 namespace {
 
 constexpr uint32_t kLanes = 256;
-constexpr uint32_t kXcds = 8;  // workgroups are dealt to the 8 XCDs round-robin
 
 __device__ __forceinline__ uint32_t live_popcount(uint64_t w0, uint64_t w1, uint64_t s,
                                                   uint64_t n_bp) {
@@ -48,6 +47,8 @@ __device__ __forceinline__ void store_pair(uint64_t *p, uint64_t x, uint64_t y) 
     __builtin_nontemporal_store(v, reinterpret_cast<u64x2 *>(p));
 }
 
+// One 64-bit atomic per wave: the whole wave takes part (no lane is
+// inactive here), lane 0 adds the wave's total.
 __device__ __forceinline__ void tally_wave(unsigned long long *ones, uint32_t slot, uint32_t pc) {
     pc = wave_sum(pc);
     if ((threadIdx.x & 63u) == 0 && pc) atomicAdd(ones + slot, (unsigned long long)pc);
@@ -97,6 +98,7 @@ struct Task {
     uint64_t n_bp;        // live words per row
     uint32_t lane_off;    // 16 * lane
     uint32_t chunk_bytes; // 8 * chunk words
+    bool full;            // every word of the chunk is below n_bp (no padding)
 };
 
 constexpr int32_t kBufferWord3 = 0x00020000;  // raw buffer, 32-bit format fields (gfx9)
@@ -118,6 +120,12 @@ __device__ __forceinline__ Pair load_k(__amdgpu_buffer_rsrc_t r, const Task &t, 
 
 __device__ __forceinline__ uint64_t lane_word(const Task &t, int k, int h) {
     return t.base + 2u * threadIdx.x + 512u * uint32_t(k) + uint32_t(h);
+}
+
+// Popcount of the lane's pair k, counting only words below n_bp.
+__device__ __forceinline__ uint32_t pair_popcount(const Task &t, const Pair &p, int k) {
+    if (t.full) return uint32_t(__popcll(p.x)) + uint32_t(__popcll(p.y));
+    return live_popcount(p.x, p.y, lane_word(t, k, 0), t.n_bp);
 }
 
 // Truth table of v_bitop3_b32 for acc <op> v0' <op> v1', v' = v or ~v; the
@@ -216,7 +224,7 @@ __device__ __forceinline__ void eval_fused(uint32_t ref, const Task &t, Pair (&v
 #pragma unroll
     for (int k = 0; k < K; ++k) {
         if (invert) { v[k].x = ~v[k].x; v[k].y = ~v[k].y; }
-        pc += live_popcount(v[k].x, v[k].y, lane_word(t, k0 + k, 0), t.n_bp);
+        pc += pair_popcount(t, v[k], k0 + k);
     }
     if (c.type_k_flags & kGateTallied) tally_wave(t.ones, c.out_row, pc);
 }
@@ -391,7 +399,7 @@ __device__ __forceinline__ void gate_task_k(const GateDesc &d, const uint32_t *_
     if (d.type_k_flags & kGateTallied) {
         uint32_t pc = 0;
 #pragma unroll
-        for (int k = 0; k < K; ++k) pc += live_popcount(r[k].x, r[k].y, lane_word(t, k, 0), t.n_bp);
+        for (int k = 0; k < K; ++k) pc += pair_popcount(t, r[k], k);
         tally_wave(t.ones, d.out_row, pc);
     }
 }
@@ -404,8 +412,9 @@ __device__ __forceinline__ void gate_task(const PassLaunch &L, uint64_t *__restr
     const uint32_t gi = uint32_t(task - uint64_t(chunk) * L.n_gates);
     const GateDesc d = L.gates[gi];
     const uint32_t *a = L.arg_rows + d.arg_begin;
-    const Task t{&L, words, ones, geo.stride, uint64_t(chunk) * geo.chunk_words, geo.n_bp,
-                 16u * threadIdx.x, 8u * geo.chunk_words};
+    const uint64_t base = uint64_t(chunk) * geo.chunk_words;
+    const Task t{&L, words, ones, geo.stride, base, geo.n_bp, 16u * threadIdx.x,
+                 8u * geo.chunk_words, base + geo.chunk_words <= geo.n_bp};
     switch (geo.chunk_words) {
     case 2048: gate_task_k<4>(d, a, t); break;
     case 1024: gate_task_k<2>(d, a, t); break;
@@ -413,41 +422,22 @@ __device__ __forceinline__ void gate_task(const PassLaunch &L, uint64_t *__restr
     }
 }
 
-// Per-XCD split of a launch: workgroup b runs on XCD b % 8 as its (b / 8)-th
-// workgroup there. XCD x owns the x-th eighth of the gate tasks (chunk-major,
-// so the gates it evaluates together read the same chunk of their input rows
-// through its own L2) and the x-th eighth of the sampler tasks, and deals
-// them out interleaved in proportion.
-struct Split {
-    uint64_t g_lo, s_lo, ngx, tx;
-};
-
-__host__ __device__ __forceinline__ Split split_for(uint64_t ng, uint64_t ns, uint32_t x) {
-    Split s;
-    s.g_lo = ng * x / kXcds;
-    s.s_lo = ns * x / kXcds;
-    s.ngx = ng * (x + 1) / kXcds - s.g_lo;
-    s.tx = s.ngx + (ns * (x + 1) / kXcds - s.s_lo);
-    return s;
-}
-
 }  // namespace
 
 __global__ __launch_bounds__(256) void pass_layer(PassLaunch L, uint64_t *__restrict__ words,
                                                   unsigned long long *__restrict__ ones,
-                                                  LaunchGeometry geo) {
+                                                  LaunchGeometry geo, SplitFracs sf) {
     const uint64_t ng = uint64_t(L.n_gates) * geo.n_chunks;
     const uint64_t ns = L.samp_end - L.samp_begin;
     const uint32_t x = blockIdx.x % kXcds;
     const uint64_t j = blockIdx.x / kXcds;
-    const Split sp = split_for(ng, ns, x);
-    if (j >= sp.tx) return;
-    const uint64_t before = j * sp.ngx / sp.tx;
-    const uint64_t upto = (j + 1) * sp.ngx / sp.tx;
-    if (upto > before)
-        gate_task(L, words, ones, geo, sp.g_lo + before);
+    const XcdShare sh = xcd_share(ng, ns, x);
+    if (j >= sh.tx) return;
+    const uint64_t before = gates_before(j, sh.ngx, sf.frac[x]);
+    if (gates_before(j + 1, sh.ngx, sf.frac[x]) > before)
+        gate_task(L, words, ones, geo, sh.g_lo + before);
     else
-        sampler_task(L, words, geo, L.samp_begin + sp.s_lo + (j - before));
+        sampler_task(L, words, geo, L.samp_begin + sh.s_lo + (j - before));
 }
 
 uint64_t pass_grid_blocks(const PassLaunch &l, const LaunchGeometry &geo) {
@@ -455,7 +445,7 @@ uint64_t pass_grid_blocks(const PassLaunch &l, const LaunchGeometry &geo) {
     const uint64_t ns = l.samp_end - l.samp_begin;
     uint64_t m = 0;
     for (uint32_t x = 0; x < kXcds; ++x) {
-        const uint64_t t = split_for(ng, ns, x).tx;
+        const uint64_t t = xcd_share(ng, ns, x).tx;
         m = t > m ? t : m;
     }
     return m * kXcds;
@@ -465,8 +455,16 @@ hipError_t launch_pass(hipStream_t stream, const PassLaunch &l, uint64_t *words,
                        unsigned long long *ones, const LaunchGeometry &geo) {
     const uint64_t blocks = pass_grid_blocks(l, geo);
     if (blocks == 0) return hipSuccess;
+    if (blocks >= (uint64_t(1) << 24)) return hipErrorInvalidConfiguration;
+    const uint64_t ng = uint64_t(l.n_gates) * geo.n_chunks;
+    const uint64_t ns = l.samp_end - l.samp_begin;
+    SplitFracs sf;
+    for (uint32_t x = 0; x < kXcds; ++x) {
+        const XcdShare sh = xcd_share(ng, ns, x);
+        sf.frac[x] = interleave_frac(sh.ngx, sh.tx);
+    }
     hipLaunchKernelGGL(pass_layer, dim3(uint32_t(blocks)), dim3(kLanes), 0, stream, l, words, ones,
-                       geo);
+                       geo, sf);
     return hipGetLastError();
 }
 

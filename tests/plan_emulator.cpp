@@ -13,6 +13,7 @@
 // gates recomputed as mcs_engine_read_words does) and every node's tally.
 //
 // Usage: plan_emulator PDAG_FILE EVENT_WORDS_FILE FUSE SPLIT_FAN_IN OUT_FILE
+//        plan_emulator --split   (checks the launch's workgroup-to-task map)
 //   PDAG_FILE: i32 n_vars, i32 n_gates, i64 n_args, i32 root, then the
 //   mcs_pdag arrays in order; EVENT_WORDS_FILE: n_events x 64 uint64 words in
 //   event_idx order. OUT_FILE: gate words (plan order, 64 each), then one
@@ -124,7 +125,58 @@ struct Emulator {
 
 }  // namespace
 
+// Every workgroup of a launch maps to one task, and every gate and sampler
+// task of the launch to exactly one workgroup (xcd_share, gates_before).
+int check_split() {
+    uint64_t state = 0x9E3779B97F4A7C15ull;
+    auto next = [&]() {
+        state ^= state << 13;
+        state ^= state >> 7;
+        state ^= state << 17;
+        return state;
+    };
+    std::vector<std::pair<uint64_t, uint64_t>> cases = {{0, 0}, {0, 1}, {1, 0}, {1, 1}, {7, 0},
+                                                        {0, 7}, {8, 8}, {9, 1}, {1, 9}, {1000, 3}};
+    for (int i = 0; i < 3000; ++i) {
+        const uint64_t scale = uint64_t(1) << (next() % 21);
+        cases.push_back({next() % scale, next() % scale});
+    }
+    cases.push_back({(uint64_t(1) << 26) - 5, (uint64_t(1) << 26) + 3});
+    for (auto [ng, ns] : cases) {
+        std::vector<uint8_t> gseen(ng, 0), sseen(ns, 0);
+        uint64_t gsum = 0, ssum = 0;
+        for (uint32_t x = 0; x < kXcds; ++x) {
+            const XcdShare sh = xcd_share(ng, ns, x);
+            const uint64_t frac = interleave_frac(sh.ngx, sh.tx);
+            if (frac > (uint64_t(1) << 32)) return 1;
+            if (gates_before(sh.tx, sh.ngx, frac) != sh.ngx) return 1;
+            for (uint64_t j = 0; j < sh.tx; ++j) {
+                const uint64_t b = gates_before(j, sh.ngx, frac);
+                const uint64_t u = gates_before(j + 1, sh.ngx, frac);
+                if (u > b + 1) return 1;
+                if (u > b) {
+                    const uint64_t g = sh.g_lo + b;
+                    if (g >= ng || gseen[g]++) return 1;
+                    ++gsum;
+                } else {
+                    const uint64_t t = sh.s_lo + (j - b);
+                    if (t >= ns || sseen[t]++) return 1;
+                    ++ssum;
+                }
+            }
+        }
+        if (gsum != ng || ssum != ns) return 1;
+    }
+    std::printf("split ok: %zu cases\n", cases.size());
+    return 0;
+}
+
 int main(int argc, char **argv) {
+    if (argc == 2 && std::string(argv[1]) == "--split") {
+        const int rc = check_split();
+        if (rc) std::printf("EMULATOR ERROR: workgroup split\n");
+        return rc ? 2 : 0;
+    }
     if (argc != 6) {
         std::fprintf(stderr, "usage: plan_emulator PDAG EVENTS FUSE SPLIT OUT\n");
         return 64;

@@ -126,3 +126,11 @@ def test_large_tree_plan_replays_oracle(emulator, tmp_path):
     launches, fused = _check(emulator, tmp_path, mc.large_tree(), 1, 64)
     assert fused > 1000
     assert launches >= 2
+
+
+def test_workgroup_split_covers_every_task_once(emulator):
+    # The pass kernel's division-free map from workgroups to gate and sampler
+    # tasks (kernels.h: xcd_share, interleave_frac, gates_before).
+    r = subprocess.run([emulator, "--split"], capture_output=True, text=True)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert r.stdout.startswith("split ok")
