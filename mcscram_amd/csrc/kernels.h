@@ -123,8 +123,32 @@ __host__ __device__ inline uint64_t gates_before(uint64_t j, uint64_t ngx, uint6
     return b < ngx ? b : ngx;
 }
 
+// High 64 bits of a 64 x 64-bit product, from 32-bit halves (on wave-uniform
+// operands the compiler keeps it on the scalar unit).
+__host__ __device__ inline uint64_t mulhi64(uint64_t a, uint64_t b) {
+    const uint64_t a0 = uint32_t(a), a1 = a >> 32, b0 = uint32_t(b), b1 = b >> 32;
+    const uint64_t p00 = a0 * b0, p01 = a0 * b1, p10 = a1 * b0, p11 = a1 * b1;
+    const uint64_t mid = (p00 >> 32) + uint32_t(p01) + uint32_t(p10);
+    return p11 + (p01 >> 32) + (p10 >> 32) + (mid >> 32);
+}
+
+// floor(n / d) from the reciprocal m = floor((2^64 - 1) / d), d >= 1: the
+// estimate floor(n m / 2^64) is floor(n / d) or one less (m > 2^64/d - 1 and
+// n < 2^64), so one correction step makes it exact.
+inline uint64_t reciprocal64(uint64_t d) { return d ? ~uint64_t(0) / d : 0; }
+
+__host__ __device__ inline uint64_t div_by_reciprocal(uint64_t n, uint64_t d, uint64_t m) {
+    uint64_t q = mulhi64(n, m);
+    if (n - q * d >= d) ++q;
+    return q;
+}
+
+// Per-launch constants computed on the host, so the kernel's task decode
+// does no division.
 struct SplitFracs {
     uint64_t frac[kXcds];  // interleave_frac of each XCD's share
+    uint64_t inv_gates;    // reciprocal64(n_gates)
+    uint64_t inv_chunks;   // reciprocal64(n_chunks)
 };
 
 // Blocks of 256 lanes; the caller checks grid_blocks(...) * 256 < 2^32.

@@ -56,8 +56,9 @@ __device__ __forceinline__ void tally_wave(unsigned long long *ones, uint32_t sl
 
 // Sampler task: words [base, base + chunk_words) of one event.
 __device__ __forceinline__ void sampler_task(const PassLaunch &L, uint64_t *__restrict__ words,
-                                             const LaunchGeometry &geo, uint64_t task) {
-    const uint32_t e = uint32_t(task / geo.n_chunks);
+                                             const LaunchGeometry &geo, const SplitFracs &sf,
+                                             uint64_t task) {
+    const uint32_t e = uint32_t(div_by_reciprocal(task, geo.n_chunks, sf.inv_chunks));
     const uint32_t chunk = uint32_t(task - uint64_t(e) * geo.n_chunks);
     const EventDesc d = L.events[e];
     const uint32_t iteration =
@@ -407,8 +408,9 @@ __device__ __forceinline__ void gate_task_k(const GateDesc &d, const uint32_t *_
 // Gate task: one gate over words [base, base + chunk_words).
 __device__ __forceinline__ void gate_task(const PassLaunch &L, uint64_t *__restrict__ words,
                                           unsigned long long *__restrict__ ones,
-                                          const LaunchGeometry &geo, uint64_t task) {
-    const uint32_t chunk = uint32_t(task / L.n_gates);
+                                          const LaunchGeometry &geo, const SplitFracs &sf,
+                                          uint64_t task) {
+    const uint32_t chunk = uint32_t(div_by_reciprocal(task, L.n_gates, sf.inv_gates));
     const uint32_t gi = uint32_t(task - uint64_t(chunk) * L.n_gates);
     const GateDesc d = L.gates[gi];
     const uint32_t *a = L.arg_rows + d.arg_begin;
@@ -435,9 +437,9 @@ __global__ __launch_bounds__(256) void pass_layer(PassLaunch L, uint64_t *__rest
     if (j >= sh.tx) return;
     const uint64_t before = gates_before(j, sh.ngx, sf.frac[x]);
     if (gates_before(j + 1, sh.ngx, sf.frac[x]) > before)
-        gate_task(L, words, ones, geo, sh.g_lo + before);
+        gate_task(L, words, ones, geo, sf, sh.g_lo + before);
     else
-        sampler_task(L, words, geo, L.samp_begin + sh.s_lo + (j - before));
+        sampler_task(L, words, geo, sf, L.samp_begin + sh.s_lo + (j - before));
 }
 
 uint64_t pass_grid_blocks(const PassLaunch &l, const LaunchGeometry &geo) {
@@ -463,6 +465,8 @@ hipError_t launch_pass(hipStream_t stream, const PassLaunch &l, uint64_t *words,
         const XcdShare sh = xcd_share(ng, ns, x);
         sf.frac[x] = interleave_frac(sh.ngx, sh.tx);
     }
+    sf.inv_gates = reciprocal64(l.n_gates);
+    sf.inv_chunks = reciprocal64(geo.n_chunks);
     hipLaunchKernelGGL(pass_layer, dim3(uint32_t(blocks)), dim3(kLanes), 0, stream, l, words, ones,
                        geo, sf);
     return hipGetLastError();

@@ -126,7 +126,8 @@ struct Emulator {
 }  // namespace
 
 // Every workgroup of a launch maps to one task, and every gate and sampler
-// task of the launch to exactly one workgroup (xcd_share, gates_before).
+// task of the launch to exactly one workgroup (xcd_share, gates_before); the
+// task decode's division by reciprocal is exact.
 int check_split() {
     uint64_t state = 0x9E3779B97F4A7C15ull;
     auto next = [&]() {
@@ -166,6 +167,17 @@ int check_split() {
             }
         }
         if (gsum != ng || ssum != ns) return 1;
+    }
+    // Task decode: division by a host-computed reciprocal.
+    const uint64_t ds[] = {1, 2, 3, 7, 8, 255, 256, 1000, 65535, 65536, 40000, 131072,
+                           (uint64_t(1) << 32) - 1, uint64_t(1) << 32, (uint64_t(1) << 63) + 5,
+                           ~uint64_t(0)};
+    for (int i = 0; i < 200000; ++i) {
+        const uint64_t d = i < 16 * 64 ? ds[i % 16] : (next() >> (next() % 64)) | 1u;
+        const uint64_t n = i % 64 == 0 ? ~uint64_t(0) - uint64_t(i % 3)
+                                       : next() >> (next() % 64);
+        if (div_by_reciprocal(n, d, reciprocal64(d)) != n / d) return 1;
+        if (mulhi64(n, d) != uint64_t((static_cast<unsigned __int128>(n) * d) >> 64)) return 1;
     }
     std::printf("split ok: %zu cases\n", cases.size());
     return 0;
