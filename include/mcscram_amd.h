@@ -284,11 +284,14 @@ typedef struct mcs_batch_config {
     uint32_t max_tile_words;  /* words per tile, power of two <= 64 (0 = 64) */
     uint32_t item_words;      /* largest work item in words (0 = 4096); smaller
                                  when the batch has too few tiles to fill the GPU */
-    uint32_t use_graph;       /* 1: replay each pass as a captured hipGraph */
+    uint32_t use_graph;       /* 1: replay the fused launch as a captured hipGraph */
     uint32_t key0, key1;      /* Philox key (the reference key by default) */
     uint32_t workgroup_lanes; /* 64..1024, a multiple of 64 (0 = 1024) */
     uint32_t min_fused_tile_words; /* narrowest LDS tile for the fused launch
                                       (0 = 8; 1 = every tree that fits; 128 = none) */
+    uint32_t layered_streams; /* layered pipelines per sample shape, each with its
+                                 own words arena and HIP stream, run side by side
+                                 and beside the fused launch (0 = 2; at most 8) */
 } mcs_batch_config;
 
 typedef struct mcs_batch_info {

@@ -7,15 +7,20 @@
 //    fused_trees (fused.hip): work items of whole tiles, largest first,
 //    every tile sampled and evaluated in LDS; only tallies reach HBM.
 //  * layered: larger trees, grouped by sample shape into pipelines through
-//    pass_layer (kernels.hip). The trees of a pipeline reuse one words
+//    pass_layer (kernels.hip), layered_streams pipelines per shape with the
+//    trees dealt to them by cost. The trees of a pipeline reuse one words
 //    arena: two event areas and one gate area. Tree t's gate launches carry
 //    the sampler tasks of the next tree (into the other event area), so the
 //    Philox work of one tree runs beside the gate traffic of another; the
 //    last tree of a pass samples the first tree of the next pass.
 //
 // A pass starts with a tiny kernel that resets the fused work queue and
-// advances the device iteration counter read by both kernels; the fused
-// part can be captured once as a hipGraph and replayed.
+// advances the device iteration counter read by both kernels. Each pipeline
+// has its own stream: after that kernel the pipelines fork from the batch's
+// stream, run beside the fused launch and beside each other (one pipeline's
+// short launches and launch tails leave the CUs to the others), and join it
+// again at the end of the pass. The fused launch can be captured once as a
+// hipGraph and replayed.
 #include <hip/hip_runtime_api.h>
 
 #include <algorithm>
@@ -63,6 +68,8 @@ struct LayeredGroup {
     mcs::LaunchGeometry geo{};
     uint32_t max_ev = 0;
     std::vector<int32_t> trees;  // batch tree ids, in pipeline order
+    hipStream_t stream = nullptr;  // the pipeline's launches
+    hipEvent_t done = nullptr;     // recorded after a pass's launches
     struct Tree {
         mcs::TreeLaunchPlan lp;
         uint64_t gate_begin = 0, args_begin = 0, ev_begin = 0;
@@ -112,6 +119,7 @@ struct mcs_batch {
     uint64_t n_ones = 0;
     uint64_t device_bytes = 0;
     uint32_t *d_state = nullptr;  // [0] fused work-queue head, [1] iteration
+    hipEvent_t began = nullptr;   // recorded after a pass's first kernel
     mcs::FusedLaunch launch{};
     std::vector<void *> allocations;
 
@@ -120,9 +128,16 @@ struct mcs_batch {
             (void)hipSetDevice(device);
             (void)hipStreamSynchronize(stream);
         }
+        for (LayeredGroup &g : groups)
+            if (g.stream) (void)hipStreamSynchronize(g.stream);
         if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
         if (graph) (void)hipGraphDestroy(graph);
         for (void *p : allocations) (void)hipFree(p);
+        for (LayeredGroup &g : groups) {
+            if (g.done) (void)hipEventDestroy(g.done);
+            if (g.stream) (void)hipStreamDestroy(g.stream);
+        }
+        if (began) (void)hipEventDestroy(began);
         if (stream) (void)hipStreamDestroy(stream);
     }
 };
@@ -147,7 +162,7 @@ mcs_status enqueue_layered(mcs_batch *b, uint32_t it) {
                 pl.samp_iteration = 0;  // this pass
                 pl.samp_row0 = area * g.max_ev;
                 pl.samp_end = uint64_t(tr.n_ev) * g.geo.n_chunks;
-                MCS_HIP(mcs::launch_pass(b->stream, pl, g.d_words, g.d_ones, g.geo));
+                MCS_HIP(mcs::launch_pass(g.stream, pl, g.d_words, g.d_ones, g.geo));
             }
             // The next tree of the pipeline: the following one, or the first
             // one of the next pass.
@@ -170,7 +185,7 @@ mcs_status enqueue_layered(mcs_batch *b, uint32_t it) {
                 pl.samp_end = split[l + 1];
                 // Gate rows start at 2 max_ev; counters of this tree start at
                 // ones, so the kernel's slot (row) is offset by 2 max_ev.
-                MCS_HIP(mcs::launch_pass(b->stream, pl, g.d_words, ones - 2u * g.max_ev, g.geo));
+                MCS_HIP(mcs::launch_pass(g.stream, pl, g.d_words, ones - 2u * g.max_ev, g.geo));
             }
             g.primed_pos = npos;
             g.primed_iter = it + nit_off;
@@ -183,15 +198,23 @@ mcs_status enqueue_layered(mcs_batch *b, uint32_t it) {
 mcs_status enqueue_pass(mcs_batch *b) {
     if (b->iteration == 0xFFFFFFFFu) return fail(MCS_ERR_STATE, "iteration counter exhausted");
     const uint32_t it = b->iteration + 1;
+    MCS_HIP(mcs::launch_fused_begin_pass(b->stream, b->d_state));
+    // Fork: the pipelines start once the iteration counter has advanced.
+    if (!b->groups.empty()) MCS_HIP(hipEventRecord(b->began, b->stream));
+    for (LayeredGroup &g : b->groups) MCS_HIP(hipStreamWaitEvent(g.stream, b->began, 0));
     if (b->graph_exec) {
         MCS_HIP(hipGraphLaunch(b->graph_exec, b->stream));
-    } else {
-        MCS_HIP(mcs::launch_fused_begin_pass(b->stream, b->d_state));
-        if (b->n_fused)
-            MCS_HIP(mcs::launch_fused(b->stream, b->launch, b->grid, b->lanes, b->lds_bytes));
+    } else if (b->n_fused) {
+        MCS_HIP(mcs::launch_fused(b->stream, b->launch, b->grid, b->lanes, b->lds_bytes));
     }
     const mcs_status st = enqueue_layered(b, it);
     if (st != MCS_OK) return st;
+    // Join: the batch's stream (and the next pass's first kernel, which
+    // advances the counter the pipelines read) waits for every pipeline.
+    for (LayeredGroup &g : b->groups) {
+        MCS_HIP(hipEventRecord(g.done, g.stream));
+        MCS_HIP(hipStreamWaitEvent(b->stream, g.done, 0));
+    }
     b->iteration = it;
     return MCS_OK;
 }
@@ -211,6 +234,9 @@ mcs_status build_batch(const mcs_pdag *pdags, int32_t n_trees, const mcs_sample_
     if (cfg.item_words == 0) cfg.item_words = 4096;
     if (cfg.workgroup_lanes == 0) cfg.workgroup_lanes = 1024;
     if (cfg.min_fused_tile_words == 0) cfg.min_fused_tile_words = 8;
+    if (cfg.layered_streams == 0) cfg.layered_streams = 2;
+    if (cfg.layered_streams > 8)
+        return fail(MCS_ERR_INVALID_ARGUMENT, "layered_streams must be at most 8");
     if (cfg.max_tile_words > 64 || (cfg.max_tile_words & (cfg.max_tile_words - 1)))
         return fail(MCS_ERR_INVALID_ARGUMENT, "max_tile_words must be a power of two <= 64");
     if (cfg.lds_bytes > mcs::kMaxFusedLds)
@@ -436,18 +462,74 @@ mcs_status build_batch(const mcs_pdag *pdags, int32_t n_trees, const mcs_sample_
     }
 
     // ---- layered pipelines --------------------------------------------
-    std::map<uint64_t, int32_t> group_of_shape;
-    for (int32_t t = 0; t < n_trees; ++t) {
-        TreeHost &th = b->trees[size_t(t)];
-        if (th.fused) continue;
-        auto it = group_of_shape.find(th.n_bp);
-        if (it == group_of_shape.end()) {
-            it = group_of_shape.emplace(th.n_bp, int32_t(b->groups.size())).first;
-            b->groups.emplace_back();
-            b->groups.back().n_bp = th.n_bp;
+    // Trees of one sample shape are dealt, costliest first, to the least
+    // loaded of layered_streams pipelines (cost: Philox words plus gate
+    // inputs, per word of the shape).
+    // Each pipeline's words arena holds its largest tree, so more pipelines
+    // take more memory: the count drops (to 1 at least) until every arena
+    // fits in the device's free memory.
+    std::map<uint64_t, std::vector<int32_t>> trees_of_shape;
+    for (int32_t t = 0; t < n_trees; ++t)
+        if (!b->trees[size_t(t)].fused) trees_of_shape[b->trees[size_t(t)].n_bp].push_back(t);
+    auto cost = [&](int32_t t) {
+        const TreeHost &th = b->trees[size_t(t)];
+        return double(th.plan.n_events()) * 40.0 + double(th.n_nodes);
+    };
+    // Pipeline of every tree of a shape for a given pipeline count.
+    auto deal = [&](const std::vector<int32_t> &ids, size_t n_pipes) {
+        std::vector<double> load(n_pipes, 0.0);
+        std::vector<size_t> pipe(ids.size());
+        for (size_t i = 0; i < ids.size(); ++i) {
+            pipe[i] = size_t(std::min_element(load.begin(), load.end()) - load.begin());
+            load[pipe[i]] += cost(ids[i]);
         }
-        th.group = it->second;
-        b->groups[size_t(th.group)].trees.push_back(t);
+        return pipe;
+    };
+    auto arena_bytes = [&](const std::vector<int32_t> &ids, const std::vector<size_t> &pipe,
+                           size_t n_pipes, uint64_t n_bp) {
+        std::vector<uint64_t> max_ev(n_pipes, 0), max_gate(n_pipes, 0);
+        for (size_t i = 0; i < ids.size(); ++i) {
+            const TreeHost &th = b->trees[size_t(ids[i])];
+            max_ev[pipe[i]] = std::max<uint64_t>(max_ev[pipe[i]], uint64_t(th.plan.n_events()));
+            max_gate[pipe[i]] = std::max<uint64_t>(max_gate[pipe[i]], uint64_t(th.n_nodes - th.plan.n_events()));
+        }
+        const uint64_t cw = pick_chunk_words(n_bp);
+        const uint64_t stride = (n_bp + cw - 1) / cw * cw;
+        uint64_t total = 0;
+        for (size_t k = 0; k < n_pipes; ++k) total += (2 * max_ev[k] + max_gate[k]) * stride * 8;
+        return total;
+    };
+    size_t free_now = 0, total_now = 0;
+    if (!trees_of_shape.empty()) MCS_HIP(hipMemGetInfo(&free_now, &total_now));
+    size_t n_pipes_max = cfg.layered_streams;
+    for (auto &[n_bp, ids] : trees_of_shape)
+        std::stable_sort(ids.begin(), ids.end(), [&](int32_t x, int32_t y) { return cost(x) > cost(y); });
+    for (; n_pipes_max > 1; --n_pipes_max) {
+        uint64_t need = 0;
+        for (auto &[n_bp, ids] : trees_of_shape) {
+            const size_t n = std::min(n_pipes_max, ids.size());
+            need += arena_bytes(ids, deal(ids, n), n, n_bp);
+        }
+        if (need + (uint64_t(256) << 20) * trees_of_shape.size() * n_pipes_max <= free_now) break;
+    }
+    for (auto &[n_bp, ids] : trees_of_shape) {
+        const size_t n_pipes = std::min(n_pipes_max, ids.size());
+        const std::vector<size_t> pipe = deal(ids, n_pipes);
+        const size_t first = b->groups.size();
+        for (size_t k = 0; k < n_pipes; ++k) {
+            b->groups.emplace_back();
+            b->groups.back().n_bp = n_bp;
+        }
+        for (size_t i = 0; i < ids.size(); ++i) {
+            b->trees[size_t(ids[i])].group = int32_t(first + pipe[i]);
+            b->groups[first + pipe[i]].trees.push_back(ids[i]);
+        }
+    }
+    if (!b->groups.empty())
+        MCS_HIP(hipEventCreateWithFlags(&b->began, hipEventDisableTiming));
+    for (LayeredGroup &g : b->groups) {
+        MCS_HIP(hipStreamCreateWithFlags(&g.stream, hipStreamNonBlocking));
+        MCS_HIP(hipEventCreateWithFlags(&g.done, hipEventDisableTiming));
     }
     for (LayeredGroup &g : b->groups) {
         mcs::LaunchGeometry &geo = g.geo;
@@ -527,15 +609,12 @@ mcs_status build_batch(const mcs_pdag *pdags, int32_t n_trees, const mcs_sample_
     }
     MCS_HIP(hipStreamSynchronize(b->stream));
 
-    if (cfg.use_graph) {
+    if (cfg.use_graph && b->n_fused) {
         MCS_HIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
-        hipError_t e1 = mcs::launch_fused_begin_pass(b->stream, b->d_state);
-        hipError_t e2 = b->n_fused ? mcs::launch_fused(b->stream, b->launch, b->grid, b->lanes, b->lds_bytes)
-                                   : hipSuccess;
-        hipError_t e3 = hipStreamEndCapture(b->stream, &b->graph);
+        hipError_t e1 = mcs::launch_fused(b->stream, b->launch, b->grid, b->lanes, b->lds_bytes);
+        hipError_t e2 = hipStreamEndCapture(b->stream, &b->graph);
         MCS_HIP(e1);
         MCS_HIP(e2);
-        MCS_HIP(e3);
         MCS_HIP(hipGraphInstantiate(&b->graph_exec, b->graph, nullptr, nullptr, 0));
     }
     *out = b.release();
@@ -556,6 +635,7 @@ void mcs_batch_config_default(mcs_batch_config *cfg) {
     cfg->key1 = MCS_REFERENCE_KEY1;
     cfg->workgroup_lanes = 1024;
     cfg->min_fused_tile_words = 8;
+    cfg->layered_streams = 2;
 }
 
 mcs_status mcs_batch_create(const mcs_pdag *pdags, int32_t n_trees, const mcs_sample_shape *shapes,

@@ -58,7 +58,7 @@ class _BatchConfig(C.Structure):
     _fields_ = [("lds_bytes", C.c_uint32), ("max_tile_words", C.c_uint32),
                 ("item_words", C.c_uint32), ("use_graph", C.c_uint32),
                 ("key0", C.c_uint32), ("key1", C.c_uint32), ("workgroup_lanes", C.c_uint32),
-                ("min_fused_tile_words", C.c_uint32)]
+                ("min_fused_tile_words", C.c_uint32), ("layered_streams", C.c_uint32)]
 
 
 class _BatchInfo(C.Structure):
@@ -380,7 +380,8 @@ class Batch:
     ``shapes``: one shape per tree (int words or (batch, bitpacks)), or one
     shape for all. ``tallies``: one list of PDAG indices per tree (None or an
     empty list tallies the root). Config keywords: lds_bytes,
-    max_tile_words, item_words, use_graph, key.
+    max_tile_words, item_words, use_graph, key, workgroup_lanes,
+    min_fused_tile_words, layered_streams.
     """
 
     def __init__(self, pdags, shapes, tallies=None, **config):
@@ -405,7 +406,10 @@ class Batch:
         key = config.pop("key", None)
         if key is not None:
             cfg.key0, cfg.key1 = key[0] & 0xFFFFFFFF, key[1] & 0xFFFFFFFF
+        known = {f[0] for f in _BatchConfig._fields_}
         for k, v in config.items():
+            if k not in known:
+                raise TypeError(f"unknown batch config keyword {k!r}")
             setattr(cfg, k, int(v))
         self._cfg = cfg
         handle = C.c_void_p()

@@ -25,6 +25,10 @@ def _mixed_trees():
 def test_batch_rejects_bad_config():
     with pytest.raises(EngineError, match="power of two"):
         mc.Batch([mc.small_tree(1)], 1024, max_tile_words=48)
+    with pytest.raises(EngineError, match="layered_streams"):
+        mc.Batch([mc.small_tree(1)], 1024, layered_streams=9)
+    with pytest.raises(TypeError, match="unknown batch config"):
+        mc.Batch([mc.small_tree(1)], 1024, layered_stream=2)
 
 
 def _check_batch(trees, shapes, n_pass=1, **cfg):
@@ -55,7 +59,10 @@ def test_batch_matches_oracle_per_tree(gpu):
                                  dict(max_tile_words=1, min_fused_tile_words=1),
                                  dict(max_tile_words=16, item_words=100000),
                                  dict(workgroup_lanes=64), dict(workgroup_lanes=256, lds_bytes=150 << 10),
-                                 dict(min_fused_tile_words=128), dict(lds_bytes=4096)])
+                                 dict(min_fused_tile_words=128), dict(lds_bytes=4096),
+                                 dict(layered_streams=1),
+                                 dict(layered_streams=3, min_fused_tile_words=128),
+                                 dict(layered_streams=8, min_fused_tile_words=128, use_graph=0)])
 def test_batch_configs_give_identical_tallies(gpu, cfg):
     # Fused only, layered only and mixed batches; tile widths from 1 to 64.
     trees = _mixed_trees()[:5]
