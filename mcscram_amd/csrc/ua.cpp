@@ -3,11 +3,13 @@
 //
 // The samples share the tree's plan, schedule, gate descriptors and
 // argument rows; each has its own event descriptors (thresholds) and Philox
-// key. Words arena: two event areas (samples alternate between them) and one
-// gate area, `stride` words per row. Sample k's gate launches carry the
-// sampler tasks of sample k + 1, so Philox work and gate traffic overlap
-// across samples; the last sample of a pass samples the first sample of the
-// next pass.
+// key. The samples are split into up to two pipelines, each with its own
+// HIP stream and words arena (two event areas, which its samples alternate
+// between, and one gate area, `stride` words per row), so one pipeline's
+// short launches and launch tails leave the CUs to the other. Within a
+// pipeline, sample k's gate launches carry the sampler tasks of sample k + 1,
+// so Philox work and gate traffic overlap across samples; the pipeline's last
+// sample of a pass samples its first sample of the next pass.
 #include <hip/hip_runtime_api.h>
 
 #include <algorithm>
@@ -29,12 +31,23 @@ namespace {
 
 constexpr uint32_t kUaSplitFanIn = 64;  // as the single-tree engine
 constexpr uint32_t kUaTaskRows = 64;
+constexpr uint32_t kUaPipes = 2;        // pipelines when the memory allows
 
 uint32_t pick_chunk_words(uint64_t n_bp) {
     if (n_bp >= (uint64_t(1) << 18)) return 2048;
     if (n_bp >= (uint64_t(1) << 16)) return 1024;
     return 512;
 }
+
+struct UaPipe {
+    hipStream_t stream = nullptr;
+    hipEvent_t done = nullptr;           // recorded after a run's launches
+    uint64_t *d_words = nullptr;         // the pipeline's words arena
+    uint32_t k_begin = 0, k_end = 0;     // its samples
+    uint64_t seq = 0;                    // sample-passes run: event area = seq & 1
+    int64_t primed = -1;                 // sample sampled into area seq & 1 ...
+    uint32_t primed_iter = 0;            // ... for this iteration
+};
 
 }  // namespace
 
@@ -52,10 +65,8 @@ struct mcs_ua {
     std::vector<int32_t> tally_nodes;    // plan node ids
     uint64_t slots = 0;                  // counters per sample (gates, partials, events)
     uint32_t iteration = 0;
-    uint64_t seq = 0;                    // sample-passes run: event area = seq & 1
-    int64_t primed = -1;                 // sample sampled into area seq & 1 ...
-    uint32_t primed_iter = 0;            // ... for this iteration
-    uint64_t *d_words = nullptr;
+    std::vector<UaPipe> pipes;
+    hipEvent_t began = nullptr;          // recorded before a run's launches
     mcs::EventDesc *d_events = nullptr;  // n_samples x n_ev
     mcs::GateDesc *d_gates = nullptr;
     uint32_t *d_args[2] = {nullptr, nullptr};
@@ -67,10 +78,18 @@ struct mcs_ua {
             (void)hipSetDevice(device);
             (void)hipStreamSynchronize(stream);
         }
-        for (void *p : {static_cast<void *>(d_words), static_cast<void *>(d_events),
-                        static_cast<void *>(d_gates), static_cast<void *>(d_args[0]),
-                        static_cast<void *>(d_args[1]), static_cast<void *>(d_ones)})
+        for (UaPipe &q : pipes)
+            if (q.stream) (void)hipStreamSynchronize(q.stream);
+        for (void *p : {static_cast<void *>(d_events), static_cast<void *>(d_gates),
+                        static_cast<void *>(d_args[0]), static_cast<void *>(d_args[1]),
+                        static_cast<void *>(d_ones)})
             if (p) (void)hipFree(p);
+        for (UaPipe &q : pipes) {
+            if (q.d_words) (void)hipFree(q.d_words);
+            if (q.done) (void)hipEventDestroy(q.done);
+            if (q.stream) (void)hipStreamDestroy(q.stream);
+        }
+        if (began) (void)hipEventDestroy(began);
         if (stream) (void)hipStreamDestroy(stream);
     }
 };
@@ -78,50 +97,59 @@ struct mcs_ua {
 namespace {
 
 mcs_status enqueue_run(mcs_ua *u, uint32_t n_passes) {
-    const uint32_t K = uint32_t(u->n_samples);
     const uint64_t samp_total = uint64_t(u->n_ev) * u->geo.n_chunks;
     const std::vector<uint64_t> split = mcs::split_in_proportion(u->lp.launch_cost, samp_total);
+    // Fork: the pipelines start after what the engine's stream holds.
+    MCS_HIP(hipEventRecord(u->began, u->stream));
+    for (UaPipe &q : u->pipes) MCS_HIP(hipStreamWaitEvent(q.stream, u->began, 0));
     for (uint32_t p = 0; p < n_passes; ++p) {
         if (u->iteration == 0xFFFFFFFFu) return fail(MCS_ERR_STATE, "iteration counter exhausted");
         const uint32_t it = u->iteration + 1;
-        for (uint32_t k = 0; k < K; ++k) {
-            const uint32_t area = uint32_t(u->seq & 1u);
-            mcs::PassLaunch base{};
-            base.key1 = u->cfg.key1;
-            if (!(u->primed == int64_t(k) && u->primed_iter == it)) {
-                mcs::PassLaunch pl = base;
-                pl.events = u->d_events + uint64_t(k) * u->n_ev;
-                pl.key0 = u->key0[k];
-                pl.samp_iteration = it;
-                pl.samp_row0 = area * u->n_ev;
-                pl.samp_end = samp_total;
-                MCS_HIP(mcs::launch_pass(u->stream, pl, u->d_words, u->d_ones, u->geo));
-            }
-            const uint32_t nk = k + 1 < K ? k + 1 : 0;
-            const uint32_t nit = k + 1 < K ? it : it + 1;
-            const bool prefetch = nit != 0;  // iteration counter not wrapping
-            unsigned long long *ones = u->d_ones + uint64_t(k) * u->slots;  // + pad - 2 n_ev
-            for (size_t l = 0; l + 1 < u->lp.launch_begin.size(); ++l) {
-                mcs::PassLaunch pl = base;
-                pl.gates = u->d_gates + u->lp.launch_begin[l];
-                pl.tree_gates = u->d_gates;
-                pl.arg_rows = u->d_args[area];
-                pl.n_gates = u->lp.launch_begin[l + 1] - u->lp.launch_begin[l];
-                if (prefetch) {
-                    pl.events = u->d_events + uint64_t(nk) * u->n_ev;
-                    pl.key0 = u->key0[nk];
-                    pl.samp_iteration = nit;
-                    pl.samp_row0 = (area ^ 1u) * u->n_ev;
-                    pl.samp_begin = split[l];
-                    pl.samp_end = split[l + 1];
+        for (UaPipe &q : u->pipes) {
+            for (uint32_t k = q.k_begin; k < q.k_end; ++k) {
+                const uint32_t area = uint32_t(q.seq & 1u);
+                mcs::PassLaunch base{};
+                base.key1 = u->cfg.key1;
+                if (!(q.primed == int64_t(k) && q.primed_iter == it)) {
+                    mcs::PassLaunch pl = base;
+                    pl.events = u->d_events + uint64_t(k) * u->n_ev;
+                    pl.key0 = u->key0[k];
+                    pl.samp_iteration = it;
+                    pl.samp_row0 = area * u->n_ev;
+                    pl.samp_end = samp_total;
+                    MCS_HIP(mcs::launch_pass(q.stream, pl, q.d_words, u->d_ones, u->geo));
                 }
-                MCS_HIP(mcs::launch_pass(u->stream, pl, u->d_words, ones, u->geo));
+                const uint32_t nk = k + 1 < q.k_end ? k + 1 : q.k_begin;
+                const uint32_t nit = k + 1 < q.k_end ? it : it + 1;
+                const bool prefetch = nit != 0;  // iteration counter not wrapping
+                unsigned long long *ones = u->d_ones + uint64_t(k) * u->slots;  // + pad - 2 n_ev
+                for (size_t l = 0; l + 1 < u->lp.launch_begin.size(); ++l) {
+                    mcs::PassLaunch pl = base;
+                    pl.gates = u->d_gates + u->lp.launch_begin[l];
+                    pl.tree_gates = u->d_gates;
+                    pl.arg_rows = u->d_args[area];
+                    pl.n_gates = u->lp.launch_begin[l + 1] - u->lp.launch_begin[l];
+                    if (prefetch) {
+                        pl.events = u->d_events + uint64_t(nk) * u->n_ev;
+                        pl.key0 = u->key0[nk];
+                        pl.samp_iteration = nit;
+                        pl.samp_row0 = (area ^ 1u) * u->n_ev;
+                        pl.samp_begin = split[l];
+                        pl.samp_end = split[l + 1];
+                    }
+                    MCS_HIP(mcs::launch_pass(q.stream, pl, q.d_words, ones, u->geo));
+                }
+                q.primed = prefetch ? int64_t(nk) : -1;
+                q.primed_iter = nit;
+                ++q.seq;
             }
-            u->primed = prefetch ? int64_t(nk) : -1;
-            u->primed_iter = nit;
-            ++u->seq;
         }
         u->iteration = it;
+    }
+    // Join: the engine's stream waits for every pipeline.
+    for (UaPipe &q : u->pipes) {
+        MCS_HIP(hipEventRecord(q.done, q.stream));
+        MCS_HIP(hipStreamWaitEvent(u->stream, q.done, 0));
     }
     return MCS_OK;
 }
@@ -220,13 +248,25 @@ mcs_status mcs_ua_create(const mcs_pdag *pdag, const mcs_sample_shape *shape, in
     const uint64_t ones_n = 2ull * u->n_ev + uint64_t(n_samples) * u->slots;
     size_t free_b = 0, total_b = 0;
     MCS_HIP(hipMemGetInfo(&free_b, &total_b));
-    const uint64_t need = word_bytes + ev.size() * sizeof(mcs::EventDesc) +
-                          u->lp.gates.size() * sizeof(mcs::GateDesc) +
-                          (u->lp.args[0].size() + u->lp.args[1].size()) * 4 + ones_n * 8;
+    const uint64_t fixed = ev.size() * sizeof(mcs::EventDesc) + u->lp.gates.size() * sizeof(mcs::GateDesc) +
+                           (u->lp.args[0].size() + u->lp.args[1].size()) * 4 + ones_n * 8;
+    // Two pipelines (one arena each) when both fit, else one.
+    uint32_t n_pipes = std::min<uint32_t>(kUaPipes, uint32_t(n_samples));
+    while (n_pipes > 1 && fixed + n_pipes * word_bytes + (uint64_t(256) << 20) > free_b) --n_pipes;
+    const uint64_t need = fixed + n_pipes * word_bytes;
     if (need + (uint64_t(256) << 20) > free_b)
         return fail(MCS_ERR_OUT_OF_MEMORY, "uncertainty engine needs " + std::to_string(need >> 20) +
                                                " MiB, " + std::to_string(free_b >> 20) + " MiB free");
-    MCS_HIP(hipMalloc(&u->d_words, word_bytes));
+    MCS_HIP(hipEventCreateWithFlags(&u->began, hipEventDisableTiming));
+    u->pipes.resize(n_pipes);
+    for (uint32_t i = 0; i < n_pipes; ++i) {
+        UaPipe &q = u->pipes[i];
+        q.k_begin = uint32_t(uint64_t(n_samples) * i / n_pipes);
+        q.k_end = uint32_t(uint64_t(n_samples) * (i + 1) / n_pipes);
+        MCS_HIP(hipStreamCreateWithFlags(&q.stream, hipStreamNonBlocking));
+        MCS_HIP(hipEventCreateWithFlags(&q.done, hipEventDisableTiming));
+        MCS_HIP(hipMalloc(&q.d_words, word_bytes));
+    }
     MCS_HIP(hipMalloc(&u->d_events, ev.size() * sizeof(mcs::EventDesc)));
     MCS_HIP(hipMalloc(&u->d_gates, std::max<size_t>(u->lp.gates.size(), 1) * sizeof(mcs::GateDesc)));
     for (int a = 0; a < 2; ++a)
