@@ -7,11 +7,13 @@ Sweeps, on the GPU, the knobs that decide how a batch of trees is executed:
   fused whole in LDS; narrower trees run layer by layer through the layered
   pipeline (one layer per launch instead of all layers in one pass);
 * ``lds_bytes`` — LDS per workgroup of the fused launch;
-* ``workgroup_lanes`` — workgroup shape of the fused launch.
+* ``workgroup_lanes`` — workgroup shape of the fused launch;
+* ``layered_streams`` — layered pipelines run side by side (each with its
+  own words arena, so it is swept at full size only).
 
 Each configuration is timed on a proxy of the workload (the same trees at
-fewer words per pass); the fastest few are re-timed at full size, and the
-winner is written to ``mcscram_amd/tuned.json`` under the workload's name,
+fewer words per pass); the fastest few are re-timed at full size with each
+``layered_streams`` value, and the winner is written to ``mcscram_amd/tuned.json`` under the workload's name,
 with the sweep that chose it. Every configuration gives bit-identical
 tallies, so the choice is about speed only.
 
@@ -34,6 +36,7 @@ SPACE = {
     "lds_bytes": [32 << 10, 64 << 10, 96 << 10],
     "workgroup_lanes": [256, 512, 1024],
 }
+STREAMS = [1, 2, 3]  # layered_streams, at full size
 
 
 def workload(name: str):
@@ -72,9 +75,11 @@ def sweep(name: str, proxy_div: int = 16, keep: int = 4, log=print) -> dict:
     results.sort(key=lambda r: r["proxy_ms"])
     finals = []
     for r in results[:keep]:
-        ms = time_config(trees, words, r["config"])
-        finals.append({"config": r["config"], "ms_per_pass": ms})
-        log(f"full  {words} words  {ms:9.3f} ms  {r['config']}")
+        for ls in STREAMS:
+            cfg = dict(r["config"], layered_streams=ls)
+            ms = time_config(trees, words, cfg)
+            finals.append({"config": cfg, "ms_per_pass": ms})
+            log(f"full  {words} words  {ms:9.3f} ms  {cfg}")
     finals.sort(key=lambda r: r["ms_per_pass"])
     return {"config": finals[0]["config"], "ms_per_pass": finals[0]["ms_per_pass"],
             "words_per_pass": words, "proxy_words_per_pass": proxy_words,
