@@ -49,6 +49,20 @@ def cpu_baseline(pd, target_s: float, max_words: int):
     return words, ones, dt, orc.max_threads()
 
 
+def host_cpu() -> dict:
+    """The host's CPU model and logical CPU count (the CPU baseline's machine)."""
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"cpu_model": model, "logical_cpus": os.cpu_count()}
+
+
 def measured_traffic(tree: str, n_bp: int):
     """HBM bytes per launch of pass_layer from the committed rocprofv3
     FETCH_SIZE / WRITE_SIZE summary of this workload (scaled by the copy
@@ -144,7 +158,7 @@ def bench_fused(args, mc) -> int:
             "value": sum(trees[i].n_gates for i in sub) * sw * 64 / dt, "unit": "gate-evals/s",
             "cores": orc.max_threads(), "kind": "port",
             "sample": f"oracle, trees {sub}, one pass of {sw} words each, {dt:.1f} s",
-            "parity_bit_exact": ok}
+            "parity_bit_exact": ok, **host_cpu()}
     b.close()
     print(json.dumps(out), flush=True)
     return 0
@@ -347,7 +361,7 @@ def main() -> int:
             "kind": "port",
             "sample": f"oracle, same tree, one pass of {words} words ({words * 64} trials), "
                       f"every gate tallied, {dt:.1f} s on {threads} OpenMP threads",
-            "parity_bit_exact": [int(x) for x in ones] == gpu_ones}
+            "parity_bit_exact": [int(x) for x in ones] == gpu_ones, **host_cpu()}
     if args.copy_calibration:
         out["copy_calibration_bytes"] = copy_calibration(1 << 30)
     print(json.dumps(out), flush=True)
