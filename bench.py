@@ -245,7 +245,9 @@ def tuned_batch_config(workload: str) -> dict:
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=32)
+    ap.add_argument("--steps", type=int, default=0,
+                    help="timed passes (default: 2^32 trials for the layered tree, i.e. 256 "
+                         "passes at 2^18 words; 3 for batched; 32 for small-fused)")
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--words", type=int, default=1 << 18, help="uint64 words per pass")
     ap.add_argument("--tree", choices=["large", "small"], default="large")
@@ -276,6 +278,11 @@ def main() -> int:
               "design note, not a launch path; run with --gpus 1", file=sys.stderr)
         return 2
 
+    if args.steps <= 0:
+        if args.workload == "layered":
+            args.steps = max(1, (1 << 32) // (args.words * 64))
+        else:
+            args.steps = 3 if args.workload == "batched" else 32
     import mcscram_amd as mc
     if args.workload in ("ua", "ua-large"):
         return bench_ua(args)
