@@ -234,7 +234,7 @@ Schedule build_schedule(const Plan &plan, uint32_t max_fan_in) {
         if (pg.type == MCS_OR || pg.type == MCS_NOR) base = MCS_OR;
         if (pg.type == MCS_XOR) base = MCS_XOR;
         if (base == 0xFF || n_in <= cap) {
-            place(ExecGate{node, pg.type, pg.k, pg.pos_in, pg.neg_in});
+            place(ExecGate{node, pg.type, pg.k, pg.pos_in, pg.neg_in, {}});
             continue;
         }
         // Literals (node, negated), shallow first; stable so equal depths keep
@@ -248,7 +248,7 @@ Schedule build_schedule(const Plan &plan, uint32_t max_fan_in) {
         while (lits.size() > cap) {
             std::vector<std::pair<int32_t, bool>> next;
             for (size_t b = 0; b < lits.size(); b += cap) {
-                ExecGate part{n_nodes + sch.n_aux++, base, 0, {}, {}};
+                ExecGate part{n_nodes + sch.n_aux++, base, 0, {}, {}, {}};
                 for (size_t j = b; j < std::min(lits.size(), b + cap); ++j)
                     (lits[j].second ? part.neg_in : part.pos_in).push_back(lits[j].first);
                 next.push_back({part.out, false});
@@ -256,7 +256,7 @@ Schedule build_schedule(const Plan &plan, uint32_t max_fan_in) {
             }
             lits = std::move(next);
         }
-        ExecGate top{node, pg.type, pg.k, {}, {}};
+        ExecGate top{node, pg.type, pg.k, {}, {}, {}};
         for (const auto &l : lits) (l.second ? top.neg_in : top.pos_in).push_back(l.first);
         place(std::move(top));
     }
