@@ -16,7 +16,7 @@
    Monte Carlo. Routed samples take the Monte Carlo value; the audit measures
    the surrogate's error on samples it was trusted with.
 
-Plain Monte Carlo over every outer sample is the fallback and the judge of
+Plain Monte Carlo over every outer sample is the fallback and the yardstick of
 the result (``plain_uncertainty``). Monte Carlo estimates use
 p = (ones + 1/2) / (trials + 1) so that the logit stays finite.
 """
