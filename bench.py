@@ -368,7 +368,9 @@ def main() -> int:
             "kind": "port",
             "sample": f"oracle, same tree, one pass of {words} words ({words * 64} trials), "
                       f"every gate tallied, {dt:.1f} s on {threads} OpenMP threads",
-            "parity_bit_exact": [int(x) for x in ones] == gpu_ones, **host_cpu()}
+            "parity_bit_exact": [int(x) for x in ones] == gpu_ones,
+            # Linear in trials: the same oracle over the whole 2^32 trials.
+            "extrapolated_seconds_2e32_trials": dt * float(1 << 32) / (words * 64), **host_cpu()}
     if args.copy_calibration:
         out["copy_calibration_bytes"] = copy_calibration(1 << 30)
     print(json.dumps(out), flush=True)
