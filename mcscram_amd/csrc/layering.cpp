@@ -7,6 +7,14 @@
 #include <cstdlib>
 #include <unordered_map>
 
+// The reference orders nodes with libstdc++'s std::sort (two unstable sorts
+// below). Its order among equal keys fixes every basic event's position, and
+// so its Philox counters: built against another C++ library the engine would
+// sample other streams, so such a build is refused.
+#ifndef __GLIBCXX__
+#error "layering.cpp needs libstdc++: its std::sort fixes the reference's event order"
+#endif
+
 namespace mcs {
 namespace {
 
