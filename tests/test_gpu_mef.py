@@ -138,3 +138,22 @@ def test_convergence_csv_row_of_a_gpu_run(gpu, tmp_path):
     assert int(row["shaper_num_iterations"]) == passes == 4
     assert row["device_type"] == "gpu" and int(row["max_compute_units"]) > 0
     assert float(row["mean"]) == pytest.approx(p, rel=1e-7)
+
+
+ALL_ARALIA = sorted(f[:-7] for f in os.listdir(os.path.join(GOLD, "aralia")) if f.endswith(".xml.gz"))
+
+
+@pytest.mark.parametrize("name", ALL_ARALIA)
+def test_aralia_model_tallies_bit_exact(gpu, name):
+    # Every basic event and gate of the reference's own models, loaded from
+    # their MEF files: the engine's one-bit counts equal the oracle's over
+    # two passes of a ragged word count (ATLEAST gates, complemented inputs,
+    # wide gates split into partial folds, fused single-consumer gates).
+    pd = mef.load_pdag(gzip.open(os.path.join(GOLD, "aralia", name + ".xml.gz")))
+    tally = [int(v) for v in pd.var_index] + [int(g) for g in pd.gate_index]
+    with mc.LayerManager(pd, 1000, tally=tally) as eng:
+        eng.pass_(2)
+        got = {t.index: t.num_one_bits for t in eng.collect_tallies()}
+    want, _ = orc.run(pd, 1000, n_iters=2, tally=tally)
+    bad = [(i, got.get(i), int(w)) for i, w in zip(tally, want) if got.get(i) != int(w)]
+    assert not bad, bad[:5]
