@@ -146,3 +146,19 @@ def test_edge_tree_oracle_against_enumeration():
         m = int(o) / (1 << 19)
         se = max(np.sqrt(m * (1 - m) / (1 << 19)), 1 / (1 << 19))
         assert abs(m - exact[idx]) <= 3.5 * se, (idx, m, exact[idx])
+
+
+@pytest.mark.gpu
+def test_batch_of_all_aralia_models_bit_exact(gpu):
+    # The reference's own models as one batch: the small ones fused in LDS,
+    # the large ones dealt to two layered pipelines.
+    import gzip
+    import os
+    from mcscram_amd import mef
+    gdir = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "mef", "aralia")
+    names = sorted(f for f in os.listdir(gdir) if f.endswith(".xml.gz"))
+    trees = [mef.load_pdag(gzip.open(os.path.join(gdir, f))) for f in names]
+    with mc.Batch(trees, 1000, [_all_nodes(t) for t in trees]) as b:
+        info = b.info()
+    assert info["num_fused_trees"] > 0 and info["num_layered_trees"] > 0
+    _check_batch(trees, [1000] * len(trees), n_pass=2)
