@@ -134,3 +134,30 @@ def test_workgroup_split_covers_every_task_once(emulator):
     r = subprocess.run([emulator, "--split"], capture_output=True, text=True)
     assert r.returncode == 0, r.stdout + r.stderr
     assert r.stdout.startswith("split ok")
+
+
+@pytest.fixture(scope="module")
+def sanitized_emulator(tmp_path_factory):
+    exe = str(tmp_path_factory.mktemp("emu_asan") / "plan_emulator")
+    cmd = ["g++", "-O1", "-g", "-std=c++17", "-fsanitize=address,undefined", "-fno-omit-frame-pointer",
+           "-fno-sanitize-recover=undefined", "-D__HIP_PLATFORM_AMD__",
+           "-I", os.path.join(ROOT, "include"), "-I", CSRC, "-I", "/opt/rocm/include",
+           os.path.join(ROOT, "tests", "plan_emulator.cpp"), os.path.join(CSRC, "layering.cpp"),
+           os.path.join(CSRC, "launch_plan.cpp"), "-o", exe]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0 and "sanitizer" in r.stderr.lower():
+        pytest.skip("no sanitizer runtime for g++ here")
+    assert r.returncode == 0, r.stderr[-3000:]
+    return exe
+
+
+def test_host_plan_code_under_address_and_ub_sanitizers(sanitized_emulator, tmp_path):
+    # The schedule, split, fusion and launch-plan code built with ASan and
+    # UBSan: any out-of-range access or undefined behaviour aborts the run.
+    r = subprocess.run([sanitized_emulator, "--split"], capture_output=True, text=True)
+    assert r.returncode == 0, r.stdout + r.stderr
+    for pd in (mc.small_tree(1), small_random(3),
+               mc.random_layered_tree(7, 60, 80, 6, all_connectives=True, neg_frac=0.2),
+               mc.mef.load_pdag(gzip.open(os.path.join(ARALIA, "baobab1.xml.gz")))):
+        for fuse, split in ((0, 64), (1, 64), (1, 3)):
+            _check(sanitized_emulator, tmp_path, pd, fuse, split)
