@@ -190,10 +190,11 @@ def bench_ua(args) -> int:
         words = args.ua_words or 1 << 16
         name = "surrogate-screened UA, 2,000 events / 3,000 gates"
     train_words = args.ua_train_words or 1 << 16
+    epochs = args.ua_epochs or (60 if large else 30)
     say = lambda m: print(f"[ua] {m}", file=sys.stderr, flush=True)  # noqa: E731
     say(f"training pairs: {args.ua_train} x {train_words * 64} trials")
     ens, info = scr.train_surrogate(pd, model, args.ua_train, train_words, seed=3,
-                                    epochs=args.ua_epochs,
+                                    epochs=epochs,
                                     progress=lambda d, n: say(f"training pairs {d}/{n}"))
     say(f"trained: {info}")
     res = scr.screened_uncertainty(pd, model, ens, outer, words, seed=1234)
@@ -211,7 +212,7 @@ def bench_ua(args) -> int:
         "config": {"workload": name, "training_pairs": args.ua_train,
                    "trials_per_training_pair": train_words * 64, "outer_samples": outer,
                    "trials_per_mc_run": words * 64,
-                   "network": f"{pd.n_vars}-512-512-512-1 GELU x2",
+                   "network": f"{pd.n_vars}-512-512-512-1 GELU x2", "epochs": epochs,
                    "disagreement": 0.05, "audit": 0.05},
         "plain_mc_samples_per_s": outer / t_plain,
         "speedup_vs_plain_mc": t_plain / t_scr,
@@ -266,7 +267,9 @@ def main() -> int:
                     help="words per outer Monte Carlo run (ua: 2^16; ua-large: 2^18)")
     ap.add_argument("--ua-train-words", type=int, default=0,
                     help="words per training Monte Carlo run (2^16)")
-    ap.add_argument("--ua-epochs", type=int, default=30)
+    ap.add_argument("--ua-epochs", type=int, default=0,
+                    help="training epochs (ua: 30; ua-large: 60, which halves the members' "
+                         "disagreement on its 956 inputs)")
     ap.add_argument("--trees", type=int, default=256)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
