@@ -208,29 +208,34 @@ __global__ void gelu_back(const float *__restrict__ da, const float *__restrict_
     dz[i] = to_bf16(da[i] * gelu_grad(z[i]));
 }
 
-// Column sums over rows of a BF16 [rows x h] matrix (bias gradients), and
-// gw4[j] = sum_r a3[r][j] * d[r], gb4 = sum_r d[r].
-__global__ void col_sums(const uint16_t *__restrict__ m, int32_t rows, int32_t h,
-                         float *__restrict__ out) {
-    const int j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= h) return;
-    float s = 0.f;
-    for (int r = 0; r < rows; ++r) s += from_bf16(m[int64_t(r) * h + j]);
-    out[j] = s;
-}
+// Column sums over the rows of a BF16 [rows x h] matrix, each row weighted
+// by w[r] when w is given: out[j] = sum_r w[r] m[r][j] (bias gradients, and
+// the output layer's weight gradient). A block of 16 waves covers 64
+// columns, one per lane; wave k takes rows k, k + 16, ..., so every load of
+// a wave is one coalesced 128-byte row segment, and the 16 partial sums
+// meet in LDS in wave order: the result does not depend on scheduling.
+constexpr int kColWaves = 16;
 
-__global__ void out_grads(const uint16_t *__restrict__ a3, const float *__restrict__ d, int32_t rows,
-                          int32_t h, float *__restrict__ gw4, float *__restrict__ gb4) {
-    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+__global__ __launch_bounds__(64 * kColWaves) void col_sums(const uint16_t *__restrict__ m,
+                                                           const float *__restrict__ w,
+                                                           int32_t rows, int32_t h,
+                                                           float *__restrict__ out) {
+    __shared__ float part[kColWaves][64];
+    const int lane = int(threadIdx.x) & 63, wave = int(threadIdx.x) >> 6;
+    const int j = int(blockIdx.x) * 64 + lane;
+    float s = 0.f;
     if (j < h) {
-        float s = 0.f;
-        for (int r = 0; r < rows; ++r) s += from_bf16(a3[int64_t(r) * h + j]) * d[r];
-        gw4[j] = s;
+        for (int r = wave; r < rows; r += kColWaves) {
+            const float v = from_bf16(m[int64_t(r) * h + j]);
+            s = w ? fmaf(v, w[r], s) : s + v;
+        }
     }
-    if (j == 0) {
-        float s = 0.f;
-        for (int r = 0; r < rows; ++r) s += d[r];
-        gb4[0] = s;
+    part[wave][lane] = s;
+    __syncthreads();
+    if (wave == 0 && j < h) {
+        float t = 0.f;
+        for (int k = 0; k < kColWaves; ++k) t += part[k][lane];
+        out[j] = t;
     }
 }
 
@@ -379,28 +384,30 @@ mcs_status backward(mcs_mlp *n, int32_t rows) {
     const int64_t RH = int64_t(rows) * H;
     float *g = n->g;
     const uint16_t *W2 = n->pb + n->off_w2, *W3 = n->pb + n->off_w3;
-    hipLaunchKernelGGL(mcs::out_grads, dim3(mcs::blocks_for(H)), dim3(256), 0, s, n->a3, n->d, rows,
-                       H, g + n->off_w4, g + n->off_b4);
+    // Output layer: gw4[j] = sum_r a3[r][j] d[r], gb4 = sum_r d[r].
+    hipLaunchKernelGGL(mcs::col_sums, dim3(mcs::blocks_for(H, 64)), dim3(64 * mcs::kColWaves), 0, s,
+                       n->a3, n->d, rows, H, g + n->off_w4);
+    hipLaunchKernelGGL(mcs::sum_vec, dim3(1), dim3(64), 0, s, n->d, rows, g + n->off_b4);
     hipLaunchKernelGGL(mcs::out_back, dim3(mcs::blocks_for(RH)), dim3(256), 0, s, n->d,
                        n->pb + n->off_w4, n->z3, H, RH, n->dz);
     // Layer 3: gW3 = a2^T dz3, gb3, da2 = dz3 W3^T.
     MCS_HIP(mcs::launch_gemm_bf16(s, gemm(n->a2, 1, H, n->dz, H, 1, g + n->off_w3, H, H, H, rows)));
-    hipLaunchKernelGGL(mcs::col_sums, dim3(mcs::blocks_for(H)), dim3(256), 0, s, n->dz, rows, H,
-                       g + n->off_b3);
+    hipLaunchKernelGGL(mcs::col_sums, dim3(mcs::blocks_for(H, 64)), dim3(64 * mcs::kColWaves), 0, s,
+                       n->dz, nullptr, rows, H, g + n->off_b3);
     MCS_HIP(mcs::launch_gemm_bf16(s, gemm(n->dz, H, 1, W3, 1, H, n->da, H, rows, H, H)));
     hipLaunchKernelGGL(mcs::gelu_back, dim3(mcs::blocks_for(RH)), dim3(256), 0, s, n->da, n->z2, RH,
                        n->dz);
     // Layer 2.
     MCS_HIP(mcs::launch_gemm_bf16(s, gemm(n->a1, 1, H, n->dz, H, 1, g + n->off_w2, H, H, H, rows)));
-    hipLaunchKernelGGL(mcs::col_sums, dim3(mcs::blocks_for(H)), dim3(256), 0, s, n->dz, rows, H,
-                       g + n->off_b2);
+    hipLaunchKernelGGL(mcs::col_sums, dim3(mcs::blocks_for(H, 64)), dim3(64 * mcs::kColWaves), 0, s,
+                       n->dz, nullptr, rows, H, g + n->off_b2);
     MCS_HIP(mcs::launch_gemm_bf16(s, gemm(n->dz, H, 1, W2, 1, H, n->da, H, rows, H, H)));
     hipLaunchKernelGGL(mcs::gelu_back, dim3(mcs::blocks_for(RH)), dim3(256), 0, s, n->da, n->z1, RH,
                        n->dz);
     // Layer 1: gW1 = x^T dz1 (dp x h), gb1.
     MCS_HIP(mcs::launch_gemm_bf16(s, gemm(n->xb, 1, n->dp, n->dz, H, 1, g + n->off_w1, H, n->dp, H, rows)));
-    hipLaunchKernelGGL(mcs::col_sums, dim3(mcs::blocks_for(H)), dim3(256), 0, s, n->dz, rows, H,
-                       g + n->off_b1);
+    hipLaunchKernelGGL(mcs::col_sums, dim3(mcs::blocks_for(H, 64)), dim3(64 * mcs::kColWaves), 0, s,
+                       n->dz, nullptr, rows, H, g + n->off_b1);
     MCS_HIP(hipGetLastError());
     return MCS_OK;
 }
