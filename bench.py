@@ -343,6 +343,9 @@ def main() -> int:
                    "layers": info["num_layers"], "words_per_pass": n_bp,
                    "trials_per_pass": n_bp * 64, "trials_timed": trials},
         "node_evals_per_s": (pd.n_vars + pd.n_gates) * trials / wall,
+        # The metric's second part: the dominant kernel's share of HBM peak
+        # (the roofline's achieved / peak, in percent).
+        "hbm_peak_pct": 100.0 * achieved / HBM_PEAK_GBS,
         "roofline": {"kernel": "pass_layer", "bound": "hbm", "achieved": achieved,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "traffic": measured_traffic(args.tree, n_bp),
