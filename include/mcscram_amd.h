@@ -251,6 +251,21 @@ void mcs_settings_default(mcs_settings *s);
 mcs_status mcs_plan_passes(uint64_t num_trials, const mcs_sample_shape *shape,
                            uint64_t *rounded_trials, uint64_t *passes);
 
+/* The sample shape the reference's sample_shaper picks on a device with the
+ * given limits (src/mc/queue/sample_shaper.h:59-147 on a non-NVIDIA device,
+ * with compute_closest_sample_shape_for_bitpacks, :274-334): trials rounded
+ * as in mcs_plan_passes (0 = no limit: 2^26 trials size the shape and
+ * *passes is 0); words per pass = the power-of-two product batch x bitpacks,
+ * batch <= max_batch and bitpacks <= max_bitpacks (the device's work-item
+ * size limits), closest to the rounded trials / 64 capped by
+ * max_alloc_bytes * 8 / num_nodes bits (ties to the smaller), collapsed to
+ * batch 1. Pass the reference device's limits to reproduce its shape, and
+ * with it its bits. */
+mcs_status mcs_sample_shape_for(uint64_t num_trials, uint64_t num_nodes, uint64_t max_alloc_bytes,
+                                uint64_t max_batch, uint64_t max_bitpacks,
+                                mcs_sample_shape *shape, uint64_t *rounded_trials,
+                                uint64_t *passes);
+
 /*
  * Entry point of the path: build the engine, run burn-in and then passes
  * until the fixed trial count is reached (or, with num_trials == 0, until

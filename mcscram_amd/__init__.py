@@ -10,6 +10,6 @@ from .pdag import (AND, ATLEAST, NAND, NOR, NOT, NULL, OR, XOR, Pdag,  # noqa: F
 from .engine import (Batch, LayerManager, Settings, Tally, beta_quantile,  # noqa: F401
                      convergence_update, library_path, load_library,
                      normal_quantile_two_sided, pdag_event_order, pdag_gate_depths,
-                     plan_passes, probability_analysis)
+                     plan_passes, probability_analysis, sample_shape_for)
 from . import mef  # noqa: F401  (Open-PSA MEF -> PDAG at compilation level 0)
 from . import report  # noqa: F401  (convergence.csv rows)

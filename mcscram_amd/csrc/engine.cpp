@@ -591,4 +591,51 @@ mcs_status mcs_plan_passes(uint64_t num_trials, const mcs_sample_shape *shape,
     return MCS_OK;
 }
 
+mcs_status mcs_sample_shape_for(uint64_t num_trials, uint64_t num_nodes, uint64_t max_alloc_bytes,
+                                uint64_t max_batch, uint64_t max_bitpacks,
+                                mcs_sample_shape *shape, uint64_t *rounded_trials,
+                                uint64_t *passes) {
+    if (!shape || !rounded_trials || !passes) return fail(MCS_ERR_INVALID_ARGUMENT, "null argument");
+    if (num_nodes == 0 || max_alloc_bytes == 0 || max_batch == 0 || max_bitpacks == 0)
+        return fail(MCS_ERR_INVALID_ARGUMENT, "nodes, allocation size and work-item limits must be positive");
+    if (max_alloc_bytes > (~uint64_t(0) >> 3))
+        return fail(MCS_ERR_INVALID_ARGUMENT, "allocation size too large");
+    // No trial limit: 2^26 trials size the shape (sample_shaper.h:67-72).
+    const bool unlimited = num_trials == 0;
+    const uint64_t requested = unlimited ? (uint64_t(1) << 26) : num_trials;
+    const uint64_t rem = requested % 64;  // nearest multiple of 64, ties down (:74-83)
+    const uint64_t total_bits = rem == 0 ? requested : (rem <= 32 ? requested - rem : requested + 64 - rem);
+    // Bits per node per iteration the largest allocation allows (:89).
+    const uint64_t cap_bits = (max_alloc_bytes * 8) / num_nodes;
+    const uint64_t target_bits = std::min(total_bits, cap_bits);  // :105
+    // Whole bitpacks (:340-344).
+    const uint64_t target = (target_bits + 63) / 64;
+    const uint64_t capacity = cap_bits / 64;
+    // Closest power-of-two product batch x bitpacks, ties to the smaller
+    // product (:274-334; power-of-two batch sizes always meet its sub-group
+    // alignment rule).
+    uint64_t best = 0, best_diff = ~uint64_t(0);
+    for (uint64_t bs = 1; bs <= max_batch && bs <= capacity; bs <<= 1) {
+        const uint64_t max_ss = std::min(max_bitpacks, capacity / bs);
+        for (uint64_t ss = 1; ss <= max_ss; ss <<= 1) {
+            const uint64_t prod = bs * ss;
+            const uint64_t diff = prod > target ? prod - target : target - prod;
+            if (diff < best_diff || (diff == best_diff && prod < best)) {
+                best_diff = diff;
+                best = prod;
+                if (diff == 0) break;
+            }
+            if (ss > (~uint64_t(0) >> 1)) break;
+        }
+        if (best_diff == 0 || bs > (~uint64_t(0) >> 1)) break;
+    }
+    if (best == 0) best = 1;  // no legal shape under the cap: the smallest one
+    shape->batch_size = 1;    // all the work in the last dimension (:109-113)
+    shape->bitpacks_per_batch = best;
+    *rounded_trials = total_bits;
+    const uint64_t per = best * 64;
+    *passes = unlimited ? 0 : (total_bits + per - 1) / per;  // :139-146
+    return MCS_OK;
+}
+
 }  // extern "C"

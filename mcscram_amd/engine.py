@@ -114,6 +114,7 @@ def load_library():
         "mcs_engine_restore": [vp, u32, vp, i32],
         "mcs_engine_time_passes": [vp, u32, vp, vp, vp],
         "mcs_plan_passes": [u64, vp, vp, vp],
+        "mcs_sample_shape_for": [u64, u64, u64, u64, u64, vp, vp, vp],
         "mcs_pdag_event_order": [vp, vp, i32, vp],
         "mcs_pdag_gate_depths": [vp, vp, i32],
         "mcs_probability_analysis": [vp, vp, vp, vp, i32, vp, i32, vp, vp],
@@ -333,6 +334,21 @@ def plan_passes(num_trials: int, shape) -> tuple[int, int]:
     t, n = C.c_uint64(), C.c_uint64()
     _check(load_library().mcs_plan_passes(num_trials, C.byref(sh), C.byref(t), C.byref(n)))
     return t.value, n.value
+
+
+def sample_shape_for(num_trials: int, num_nodes: int, max_alloc_bytes: int,
+                     max_work_item_sizes=(1024, 1024)):
+    """The (batch, bitpacks) shape the reference's sample_shaper picks on a
+    device whose largest allocation is max_alloc_bytes and whose work-item
+    size limits (y, z) are max_work_item_sizes, with the rounded trial count
+    and the pass count (0 when num_trials is 0: no limit).
+    -> ((batch, bitpacks), rounded_trials, passes)."""
+    sh = _Shape()
+    t, n = C.c_uint64(), C.c_uint64()
+    _check(load_library().mcs_sample_shape_for(int(num_trials), int(num_nodes), int(max_alloc_bytes),
+                                                int(max_work_item_sizes[0]), int(max_work_item_sizes[1]),
+                                                C.byref(sh), C.byref(t), C.byref(n)))
+    return (sh.batch_size, sh.bitpacks_per_batch), t.value, n.value
 
 
 def pdag_event_order(pdag: Pdag) -> np.ndarray:

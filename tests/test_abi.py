@@ -49,6 +49,46 @@ def test_plan_passes_matches_reference_rounding():
     assert mc.plan_passes(10, 4) == (0, 0)
 
 
+def _closest_pow2_shape(trials, nodes, max_alloc, limits):
+    """Closed form of the reference's shape choice: the power of two no
+    larger than the per-node capacity and the work-item limits that is
+    closest to the wanted bitpacks (ties to the smaller)."""
+    t = trials or (1 << 26)
+    r = t % 64
+    rounded = t if r == 0 else (t - r if r <= 32 else t + 64 - r)
+    cap_bits = max_alloc * 8 // nodes
+    want = -(-min(rounded, cap_bits) // 64)
+    hp2 = lambda v: 1 << (v.bit_length() - 1) if v else 0  # noqa: E731
+    top = min(cap_bits // 64, hp2(limits[0]) * hp2(limits[1]))
+    cands = [1 << k for k in range(0, 64) if (1 << k) <= top] or [1]
+    best = min(cands, key=lambda p: (abs(p - want), p))
+    passes = 0 if trials == 0 else -(-rounded // (64 * best))
+    return (1, best), rounded, passes
+
+
+def test_sample_shape_matches_the_reference_shaper():
+    # The small workload: 2^20 trials of a 320-node tree on a large device
+    # give 16,384 words in one pass, as the reference runs it.
+    assert mc.sample_shape_for(1 << 20, 320, 288 << 30) == ((1, 16384), 1 << 20, 1)
+    # No trial limit: 2^26 trials size the shape, no pass count.
+    assert mc.sample_shape_for(0, 320, 288 << 30) == ((1, 1 << 20), 1 << 26, 0)
+    # Rounding ties down; the per-node memory cap and the work-item limits bind.
+    assert mc.sample_shape_for(96, 3, 1 << 30)[1:] == (64, 1)
+    assert mc.sample_shape_for(1 << 32, 65_000, 288 << 30) == ((1, 1 << 19), 1 << 32, 128)
+    assert mc.sample_shape_for(1 << 40, 1, 1 << 40, (8, 16))[0] == (1, 128)
+    import random
+    rng = random.Random(3)
+    for _ in range(3000):
+        trials = rng.choice([0, rng.randrange(1, 1 << 12), rng.randrange(1, 1 << 40)])
+        nodes = rng.randrange(1, 200_000)
+        alloc = rng.randrange(1 << 16, 1 << 39)
+        limits = (rng.randrange(1, 4096), rng.randrange(1, 4096))
+        assert mc.sample_shape_for(trials, nodes, alloc, limits) == \
+            _closest_pow2_shape(trials, nodes, alloc, limits), (trials, nodes, alloc, limits)
+    with pytest.raises(EngineError):
+        mc.sample_shape_for(1 << 20, 0, 1 << 30)
+
+
 def test_normal_quantile_matches_scipy():
     from scipy.stats import norm
     # Acklam's approximation has relative error below 1.15e-9.
