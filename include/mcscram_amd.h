@@ -208,6 +208,14 @@ typedef struct mcs_settings {
     uint32_t key0, key1;       /* Philox key (reference key by default) */
     uint32_t ci_policy;        /* early-stop rule: MCS_CI_BAYES (reference
                                   default) or MCS_CI_WALD (src/settings.h:51, 426) */
+    uint32_t early_stop;       /* 1: stop once converged (src/settings.h:370-372,
+                                  453; default 1). num_trials == 0 implies it
+                                  (src/settings.cc:105-112); with num_trials > 0
+                                  the run stops at whichever of the trial count
+                                  and convergence comes first, as the CLI's
+                                  "-N n --early-stop" does
+                                  (targets/scram/utils/ConstructSettings.h:103-109).
+                                  "-N n" alone is early_stop = 0. */
 } mcs_settings;
 
 /* Stopping rules of the reference's convergence policy
@@ -266,12 +274,32 @@ mcs_status mcs_sample_shape_for(uint64_t num_trials, uint64_t num_nodes, uint64_
                                 mcs_sample_shape *shape, uint64_t *rounded_trials,
                                 uint64_t *passes);
 
+/* As mcs_sample_shape_for, with the reference shaper's vendor rule
+ * (src/mc/queue/sample_shaper.h:120-134): when nvidia_gpu is non-zero, words
+ * per pass = floor((max_alloc_bytes / 8) / (num_nodes * (1 + overhead_ratio)))
+ * with overhead_ratio clamped to [0, 1e6] (the reference's default is 0.05,
+ * src/settings.h:462), rounded down to a multiple of cache_line_bytes / 8
+ * words, whatever the trial count; batch 1. nvidia_gpu = 0 is
+ * mcs_sample_shape_for. The reference's only published run used an NVIDIA
+ * device, so this is the rule that reproduces its shape. Fails if no whole
+ * cache line fits per node (the reference would divide by zero). */
+mcs_status mcs_sample_shape_for_device(uint64_t num_trials, uint64_t num_nodes,
+                                       uint64_t max_alloc_bytes, uint64_t max_batch,
+                                       uint64_t max_bitpacks, int32_t nvidia_gpu,
+                                       double overhead_ratio, uint64_t cache_line_bytes,
+                                       mcs_sample_shape *shape, uint64_t *rounded_trials,
+                                       uint64_t *passes);
+
 /*
- * Entry point of the path: build the engine, run burn-in and then passes
- * until the fixed trial count is reached (or, with num_trials == 0, until
- * every tally's half-width meets ci_rel_margin under ci_policy, in linear and
- * log10 space), collect tallies and return the root's mean. Replaces ProbabilityAnalyzer<mc::DirectEval>::
- * CalculateTotalProbability / ComputeTallies
+ * Entry point of the path: build the engine, run burn-in passes until
+ * burn_in_trials are done, then passes until the fixed trial count is
+ * reached or (with early stop) every tally's half-width meets ci_rel_margin
+ * under ci_policy in linear and log10 space after a pass, whichever comes
+ * first; collect tallies and return the root's mean. Convergence is only
+ * checked after a post-burn-in pass, so early stop always runs at least one
+ * pass after burn-in (convergence_controller.h:294-328, 351-365).
+ * Replaces ProbabilityAnalyzer<mc::DirectEval>::CalculateTotalProbability /
+ * ComputeTallies
  * (src/mc/core/probability_analysis.cc:88-209) and
  * convergence_controller::run_to_convergence (convergence_controller.h:371-397).
  * observe lists extra PDAG indices to tally besides the root; out receives

@@ -249,9 +249,14 @@ mcs_status mcs_probability_analysis(const mcs_pdag *pdag, const mcs_settings *se
         mcs::set_error("output needs one record for the root and each distinct observed node");
         return MCS_ERR_INVALID_ARGUMENT;
     }
+    // total_passes is the shaper's TOTAL_ITERATIONS: 0 (no limit) when no
+    // trial count is given (src/mc/queue/sample_shaper.h:139-146). A zero
+    // trial count turns early stop on (src/settings.cc:105-112); otherwise
+    // the caller's flag decides, so a trial count with early stop stops at
+    // whichever comes first (targets/scram/utils/ConstructSettings.h:103-109).
     uint64_t rounded = 0, total_passes = 0;
     if (settings->num_trials) mcs_plan_passes(settings->num_trials, shape, &rounded, &total_passes);
-    const bool early_stop = settings->num_trials == 0;
+    const bool early_stop = settings->num_trials == 0 || settings->early_stop != 0;
     const uint64_t per_pass = info.num_bitpacks * 64ull;
     const uint32_t cap = settings->max_passes ? settings->max_passes : (1u << 20);
     const double z = mcs::normal_quantile_two_sided(settings->ci_confidence);
@@ -268,10 +273,14 @@ mcs_status mcs_probability_analysis(const mcs_pdag *pdag, const mcs_settings *se
         if (st != MCS_OK) break;
         ++iterations;
     }
-    // Steps: stop at the fixed pass count, or once the worst tally meets the
-    // policy's targets in linear and log10 space. The worst tally is the one
-    // with the widest linear half-width; the linear target is the relative
-    // margin times that tally's sample mean; both flags stay set once met
+    // Steps (convergence_controller.h:294-328): before each pass, stop if
+    // early stop is on and both flags are set, or if the fixed pass count
+    // (burn-in passes included) is reached; after each pass, set the flags
+    // once the worst tally meets the policy's targets in linear and log10
+    // space. Burn-in never sets them (:351-365), so with early stop at least
+    // one pass follows burn-in. The worst tally is the one with the widest
+    // linear half-width; the linear target is the relative margin times that
+    // tally's sample mean; both flags stay set once met
     // (convergence_controller.h:191-250, 313-323, 471-482).
     auto converged = [&]() {
         double max_eps = 0.0, max_eps_log = 0.0, target = 0.0;
@@ -288,14 +297,13 @@ mcs_status mcs_probability_analysis(const mcs_pdag *pdag, const mcs_settings *se
         eps_log_ok = eps_log_ok || (max_eps_log > 0 && max_eps_log <= delta);
         return eps_ok && eps_log_ok;
     };
-    if (st == MCS_OK && early_stop && iterations > 0) converged();
     while (st == MCS_OK && iterations < cap) {
         if (early_stop && eps_ok && eps_log_ok) break;
-        if (!early_stop && iterations >= total_passes) break;
+        if (total_passes && iterations >= total_passes) break;
         st = mcs_engine_pass_wait_collect(eng, 1, 0, rec.data(), int32_t(rec.size()));
         if (st != MCS_OK) break;
         ++iterations;
-        if (early_stop) converged();
+        converged();
     }
     if (st == MCS_OK && iterations == 0)
         st = mcs_engine_collect(eng, rec.data(), int32_t(rec.size()));

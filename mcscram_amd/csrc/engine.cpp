@@ -574,6 +574,7 @@ void mcs_settings_default(mcs_settings *s) {
     s->key0 = MCS_REFERENCE_KEY0;
     s->key1 = MCS_REFERENCE_KEY1;
     s->ci_policy = MCS_CI_BAYES;  // src/settings.h:426
+    s->early_stop = 1;            // src/settings.h:453
 }
 
 mcs_status mcs_plan_passes(uint64_t num_trials, const mcs_sample_shape *shape,
@@ -595,11 +596,24 @@ mcs_status mcs_sample_shape_for(uint64_t num_trials, uint64_t num_nodes, uint64_
                                 uint64_t max_batch, uint64_t max_bitpacks,
                                 mcs_sample_shape *shape, uint64_t *rounded_trials,
                                 uint64_t *passes) {
+    return mcs_sample_shape_for_device(num_trials, num_nodes, max_alloc_bytes, max_batch,
+                                       max_bitpacks, 0, 0.0, 0, shape, rounded_trials, passes);
+}
+
+mcs_status mcs_sample_shape_for_device(uint64_t num_trials, uint64_t num_nodes,
+                                       uint64_t max_alloc_bytes, uint64_t max_batch,
+                                       uint64_t max_bitpacks, int32_t nvidia_gpu,
+                                       double overhead_ratio, uint64_t cache_line_bytes,
+                                       mcs_sample_shape *shape, uint64_t *rounded_trials,
+                                       uint64_t *passes) {
     if (!shape || !rounded_trials || !passes) return fail(MCS_ERR_INVALID_ARGUMENT, "null argument");
     if (num_nodes == 0 || max_alloc_bytes == 0 || max_batch == 0 || max_bitpacks == 0)
         return fail(MCS_ERR_INVALID_ARGUMENT, "nodes, allocation size and work-item limits must be positive");
     if (max_alloc_bytes > (~uint64_t(0) >> 3))
         return fail(MCS_ERR_INVALID_ARGUMENT, "allocation size too large");
+    if (nvidia_gpu && (cache_line_bytes < 8 || !(overhead_ratio == overhead_ratio)))
+        return fail(MCS_ERR_INVALID_ARGUMENT,
+                    "the NVIDIA rule needs a cache line of at least one word and a numeric overhead ratio");
     // No trial limit: 2^26 trials size the shape (sample_shaper.h:67-72).
     const bool unlimited = num_trials == 0;
     const uint64_t requested = unlimited ? (uint64_t(1) << 26) : num_trials;
@@ -630,11 +644,27 @@ mcs_status mcs_sample_shape_for(uint64_t num_trials, uint64_t num_nodes, uint64_
         if (best_diff == 0 || bs > (~uint64_t(0) >> 1)) break;
     }
     if (best == 0) best = 1;  // no legal shape under the cap: the smallest one
+    if (nvidia_gpu) {
+        // NVIDIA GPUs take the largest allocation whatever the trial count:
+        // bitpacks = floor((max_alloc / 8) / (nodes (1 + overhead))), the
+        // overhead clamped to [0, 1e6], rounded down to whole cache lines
+        // (:120-134).
+        const double max_bitpacks_d = double(max_alloc_bytes) / 8.0;
+        const double effective_nodes = double(num_nodes) * (1.0 + std::clamp(overhead_ratio, 0.0, 1e6));
+        const uint64_t bitpacks = uint64_t(std::floor(max_bitpacks_d / effective_nodes));
+        const uint64_t line = uint64_t(double(cache_line_bytes) / 8.0);
+        best = bitpacks - bitpacks % line;
+        if (best == 0)
+            return fail(MCS_ERR_INVALID_ARGUMENT,
+                        "the NVIDIA rule leaves no whole cache line per node: the reference would divide by zero");
+    }
     shape->batch_size = 1;    // all the work in the last dimension (:109-113)
     shape->bitpacks_per_batch = best;
     *rounded_trials = total_bits;
-    const uint64_t per = best * 64;
-    *passes = unlimited ? 0 : (total_bits + per - 1) / per;  // :139-146
+    // Passes = ceil(total bits / bits per pass) in double precision, as the
+    // reference computes it (:139-146); exact below 2^53 bits.
+    const double per = double(best) * 64.0;
+    *passes = unlimited ? 0 : uint64_t(std::ceil(double(total_bits) / per));
     return MCS_OK;
 }
 

@@ -73,7 +73,7 @@ class _Settings(C.Structure):
     _fields_ = [("num_trials", C.c_uint64), ("burn_in_trials", C.c_uint64),
                 ("ci_confidence", C.c_double), ("ci_rel_margin", C.c_double),
                 ("max_passes", C.c_uint32), ("key0", C.c_uint32), ("key1", C.c_uint32),
-                ("ci_policy", C.c_uint32)]
+                ("ci_policy", C.c_uint32), ("early_stop", C.c_uint32)]
 
 
 class _Convergence(C.Structure):
@@ -115,6 +115,7 @@ def load_library():
         "mcs_engine_time_passes": [vp, u32, vp, vp, vp],
         "mcs_plan_passes": [u64, vp, vp, vp],
         "mcs_sample_shape_for": [u64, u64, u64, u64, u64, vp, vp, vp],
+        "mcs_sample_shape_for_device": [u64, u64, u64, u64, u64, i32, C.c_double, u64, vp, vp, vp],
         "mcs_pdag_event_order": [vp, vp, i32, vp],
         "mcs_pdag_gate_depths": [vp, vp, i32],
         "mcs_probability_analysis": [vp, vp, vp, vp, i32, vp, i32, vp, vp],
@@ -303,13 +304,19 @@ class Settings:
     max_passes: int = 0
     key: tuple = MCS_REFERENCE_KEY
     ci_policy: str = "bayes"  # src/settings.h:426
+    # Stop once converged (src/settings.h:453). num_trials == 0 implies it
+    # (src/settings.cc:105-112); with num_trials > 0 the run stops at the
+    # trial count or at convergence, whichever comes first. The CLI's "-N n"
+    # without "--early-stop" is early_stop=False
+    # (targets/scram/utils/ConstructSettings.h:103-109).
+    early_stop: bool = True
 
     def _c(self) -> _Settings:
         if self.ci_policy not in CI_POLICIES:
             raise ValueError(f"ci_policy must be one of {sorted(CI_POLICIES)}")
         return _Settings(self.num_trials, self.burn_in_trials, self.ci_confidence,
                          self.ci_rel_margin, self.max_passes, self.key[0], self.key[1],
-                         CI_POLICIES[self.ci_policy])
+                         CI_POLICIES[self.ci_policy], int(bool(self.early_stop)))
 
 
 def probability_analysis(pdag: Pdag, settings: Settings, shape, observe=()):
@@ -337,17 +344,22 @@ def plan_passes(num_trials: int, shape) -> tuple[int, int]:
 
 
 def sample_shape_for(num_trials: int, num_nodes: int, max_alloc_bytes: int,
-                     max_work_item_sizes=(1024, 1024)):
+                     max_work_item_sizes=(1024, 1024), nvidia_gpu: bool = False,
+                     overhead_ratio: float = 0.05, cache_line_bytes: int = 128):
     """The (batch, bitpacks) shape the reference's sample_shaper picks on a
     device whose largest allocation is max_alloc_bytes and whose work-item
     size limits (y, z) are max_work_item_sizes, with the rounded trial count
-    and the pass count (0 when num_trials is 0: no limit).
+    and the pass count (0 when num_trials is 0: no limit). nvidia_gpu applies
+    the shaper's rule for NVIDIA GPUs (the largest allocation split over the
+    nodes with overhead_ratio, in whole cache lines of cache_line_bytes;
+    src/mc/queue/sample_shaper.h:120-134).
     -> ((batch, bitpacks), rounded_trials, passes)."""
     sh = _Shape()
     t, n = C.c_uint64(), C.c_uint64()
-    _check(load_library().mcs_sample_shape_for(int(num_trials), int(num_nodes), int(max_alloc_bytes),
-                                                int(max_work_item_sizes[0]), int(max_work_item_sizes[1]),
-                                                C.byref(sh), C.byref(t), C.byref(n)))
+    _check(load_library().mcs_sample_shape_for_device(
+        int(num_trials), int(num_nodes), int(max_alloc_bytes), int(max_work_item_sizes[0]),
+        int(max_work_item_sizes[1]), int(bool(nvidia_gpu)), float(overhead_ratio),
+        int(cache_line_bytes), C.byref(sh), C.byref(t), C.byref(n)))
     return (sh.batch_size, sh.bitpacks_per_batch), t.value, n.value
 
 

@@ -64,7 +64,7 @@ def settings_pairs(settings, input_file: str = "", compilation_level: int = 0,
             ("req_burnin_trials", settings.burn_in_trials),
             ("req_conf_lvl", float(settings.ci_confidence)),
             ("req_delta", float(settings.ci_rel_margin)),
-            ("req_stop_on_conv", settings.num_trials == 0),
+            ("req_stop_on_conv", settings.num_trials == 0 or bool(settings.early_stop)),
             ("req_conv_pol", settings.ci_policy),
             ("req_no_kn", False),
             ("req_no_xor", False),

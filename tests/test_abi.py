@@ -89,6 +89,98 @@ def test_sample_shape_matches_the_reference_shaper():
         mc.sample_shape_for(1 << 20, 0, 1 << 30)
 
 
+def _nvidia_shape(trials, nodes, max_alloc, overhead, line):
+    """Closed form of the shaper's NVIDIA rule (src/mc/queue/sample_shaper.h:
+    120-134): the largest allocation in words, split over the nodes with the
+    overhead, rounded down to whole cache lines; passes = ceil(bits / bits
+    per pass)."""
+    t = trials or (1 << 26)
+    r = t % 64
+    rounded = t if r == 0 else (t - r if r <= 32 else t + 64 - r)
+    words = int((max_alloc / 8) // (nodes * (1 + min(max(overhead, 0.0), 1e6))))
+    words -= words % (line // 8)
+    passes = 0 if trials == 0 else -(-rounded // (64 * words))
+    return (1, words), rounded, passes
+
+
+def test_sample_shape_nvidia_rule():
+    # 89 nodes on a device with a 6 GiB largest allocation, 128-byte lines and
+    # the default 5 % overhead: floor(805306368 / 93.45) = 8,617,510 words,
+    # rounded down to 16-word lines = 8,617,504; the trial count is ignored.
+    assert mc.sample_shape_for(1 << 20, 89, 6 << 30, nvidia_gpu=True) == \
+        ((1, 8617504), 1 << 20, 1)
+    assert mc.sample_shape_for(0, 89, 6 << 30, nvidia_gpu=True) == ((1, 8617504), 1 << 26, 0)
+    # Overhead is clamped below at 0 and the alignment follows the line size.
+    assert mc.sample_shape_for(64, 1, 1 << 20, nvidia_gpu=True, overhead_ratio=-1.0,
+                               cache_line_bytes=64) == ((1, 131072), 64, 1)
+    assert mc.sample_shape_for(1 << 30, 3, 3000, nvidia_gpu=True, overhead_ratio=0.0,
+                               cache_line_bytes=8)[0] == (1, 125)
+    import random
+    rng = random.Random(5)
+    for _ in range(3000):
+        trials = rng.choice([0, rng.randrange(1, 1 << 12), rng.randrange(1, 1 << 40)])
+        nodes = rng.randrange(1, 200_000)
+        alloc = rng.randrange(1 << 24, 1 << 39)
+        overhead = rng.choice([0.0, 0.05, rng.random()])
+        line = rng.choice([32, 64, 128, 256])
+        want = _nvidia_shape(trials, nodes, alloc, overhead, line)
+        if want[0][1] == 0:
+            with pytest.raises(EngineError):
+                mc.sample_shape_for(trials, nodes, alloc, nvidia_gpu=True,
+                                    overhead_ratio=overhead, cache_line_bytes=line)
+            continue
+        assert mc.sample_shape_for(trials, nodes, alloc, nvidia_gpu=True, overhead_ratio=overhead,
+                                   cache_line_bytes=line) == want, (trials, nodes, alloc, overhead)
+    # No whole line per node: the reference would divide by zero; here an error.
+    with pytest.raises(EngineError):
+        mc.sample_shape_for(1 << 20, 1000, 1000, nvidia_gpu=True)
+    with pytest.raises(EngineError):
+        mc.sample_shape_for(1 << 20, 10, 1 << 30, nvidia_gpu=True, cache_line_bytes=4)
+
+
+def _estimates(ones, total):
+    L = load_library()
+    from mcscram_amd.engine import _Tally
+    t = _Tally()
+    t.num_one_bits, t.total_bits = ones, total
+    L.mcs_tally_point_estimates(C.byref(t))
+    return t.mean, t.std_err, list(t.ci)
+
+
+def test_point_estimates_follow_the_reference_closed_form():
+    """stats::populate_point_estimates (src/mc/stats/ci_utils.h:167-201):
+    mean = ones / total, SE = sqrt(mean (1 - mean) / total), ci =
+    {lo95, hi95, lo99, hi99} = mean -/+ z SE clamped to [0, 1], with z the
+    two-sided 95 % and 99 % normal quantiles; all zero at total 0."""
+    from scipy.stats import norm
+    z95, z99 = norm.ppf(0.975), norm.ppf(0.995)
+    # No samples yet: everything zero.
+    assert _estimates(0, 0) == (0.0, 0.0, [0.0] * 4)
+    # Edge counts: no ones and all ones give a zero-width interval at 0 and 1.
+    assert _estimates(0, 1 << 20) == (0.0, 0.0, [0.0] * 4)
+    assert _estimates(1 << 20, 1 << 20) == (1.0, 0.0, [1.0] * 4)
+    # Both clamps: a rare count's lower bounds clamp at 0, a near-certain
+    # count's upper bounds at 1; the other side stays open.
+    m, se, ci = _estimates(1, 64)
+    assert ci[0] == 0.0 and ci[2] == 0.0 and 0 < ci[1] < ci[3] < 1
+    m, se, ci = _estimates(63, 64)
+    assert ci[1] == 1.0 and ci[3] == 1.0 and 0 < ci[2] < ci[0] < 1
+    # Order and widths against the closed form with scipy's quantiles.
+    rng = np.random.default_rng(11)
+    for _ in range(2000):
+        total = int(rng.integers(1, 1 << 40))
+        ones = int(rng.integers(0, total + 1))
+        m, se, ci = _estimates(ones, total)
+        mu = ones / total
+        want_se = np.sqrt(mu * (1 - mu) / total)
+        assert m == mu
+        assert se == pytest.approx(want_se, rel=1e-15, abs=0)
+        want = [max(0.0, mu - z95 * want_se), min(1.0, mu + z95 * want_se),
+                max(0.0, mu - z99 * want_se), min(1.0, mu + z99 * want_se)]
+        assert ci == pytest.approx(want, rel=1e-14, abs=1e-300)
+        assert ci[2] <= ci[0] <= m <= ci[1] <= ci[3]
+
+
 def test_normal_quantile_matches_scipy():
     from scipy.stats import norm
     # Acklam's approximation has relative error below 1.15e-9.
@@ -146,6 +238,8 @@ def test_settings_defaults_are_the_reference_defaults():
     assert (cs.key0, cs.key1) == (382307844, 293830103)
     # Bayes is the reference's default stopping rule (src/settings.h:426).
     assert cs.ci_policy == 0 and s.ci_policy == "bayes"
+    # Early stop is on by default (src/settings.h:453).
+    assert cs.early_stop == 1 and s.early_stop is True
 
 
 def test_generators_are_deterministic_and_sized():

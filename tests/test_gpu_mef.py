@@ -112,7 +112,7 @@ def test_probability_analysis_entry_on_a_loaded_model(gpu):
     """The reference's entry point (fixed N) on baobab2, whose exact value the
     reference's README quotes next to its own GPU estimate."""
     pd = mef.load_pdag(gzip.open(os.path.join(GOLD, "aralia", "baobab2.xml.gz")))
-    s = mc.Settings(num_trials=1 << 30)
+    s = mc.Settings(num_trials=1 << 30, early_stop=False)
     p_total, tallies, passes = mc.probability_analysis(pd, s, WORDS)
     assert passes == (1 << 30) // (WORDS * 64)
     check(tallies[0].num_one_bits, tallies[0].total_bits, 7.13018e-4, 3.2905)
@@ -125,7 +125,7 @@ def test_convergence_csv_row_of_a_gpu_run(gpu, tmp_path):
     from mcscram_amd import report
     pd = mef.load_pdag(gzip.open(os.path.join(GOLD, "aralia", "baobab2.xml.gz")))
     path = str(tmp_path / "convergence.csv")
-    s = mc.Settings(num_trials=1 << 28)
+    s = mc.Settings(num_trials=1 << 28, early_stop=False)
     for _ in range(2):
         p, tallies, passes = report.probability_analysis_logged(pd, s, WORDS, path,
                                                                 input_file="baobab2.xml")

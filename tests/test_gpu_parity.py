@@ -2,6 +2,8 @@
 tallies on the same inputs (seeded synthetic trees, the reference's fixed
 Philox key), plus size-independent properties at larger sizes.
 """
+import math
+
 import numpy as np
 import pytest
 
@@ -147,7 +149,7 @@ def test_large_tree_full_size_invariants(gpu):
 @pytest.mark.parametrize("name", sorted(core_models()))
 def test_core_known_answers_on_gpu(gpu, name):
     pd, exact = core_models()[name]
-    settings = mc.Settings(num_trials=1 << 22, burn_in_trials=0)
+    settings = mc.Settings(num_trials=1 << 22, burn_in_trials=0, early_stop=False)
     p, tallies, passes = mc.probability_analysis(pd, settings, 1 << 14)
     assert passes == 4
     t = tallies[0]
@@ -191,37 +193,79 @@ def test_pass_wait_collect_intervals(gpu):
     assert ta == tb
 
 
+def _closed_form_wald_met(ones, total, delta, z):
+    """Both Wald targets of src/mc/scheduler/convergence_policy.h:28-61 from
+    their closed forms: z*SE <= delta*mu (linear) and z*SE/(mu ln 10) <= delta
+    (log10), with SE = sqrt(mu(1-mu)/n)."""
+    mu = ones / total
+    if mu <= 0.0 or mu >= 1.0:
+        return False
+    half = z * math.sqrt(mu * (1.0 - mu) / total)
+    return half <= delta * mu and half / (mu * math.log(10.0)) <= delta
+
+
+# Expected pass counts follow the reference controller's rules
+# (src/mc/scheduler/convergence_controller.h:294-397, 484-491), not the
+# product's loop: burn-in runs ceil(burn_in / trials per pass) passes and
+# never checks convergence; each later step first stops if early stop is on
+# and both flags are set, or if the pass count (burn-in included) has reached
+# the trial limit; otherwise it runs a pass and then sets the flags.
+# abc (p = 0.496) at 2^18 trials per pass meets both delta = 0.01 targets
+# already in its first pass (linear half-width 2.5e-3 <= 4.96e-3; log10
+# 2.2e-3 <= 0.01) and in every later one.
+EARLY_STOP_CASES = [
+    # (num_trials, early_stop, burn_in, delta, expected passes)
+    (0, True, 1 << 18, 0.01, 2),         # burn-in met the targets: one more pass
+    (0, True, 1 << 19, 0.01, 3),         # two burn-in passes, then one step
+    (0, True, 0, 0.01, 1),               # no burn-in: the first step converges
+    (1 << 20, False, 1 << 18, 0.01, 4),  # -N alone: the trial count decides
+    (1 << 20, True, 1 << 18, 0.01, 2),   # -N --early-stop: convergence first
+    (1 << 20, True, 1 << 18, 0.001, 4),  # -N --early-stop: the count first
+    (1 << 18, False, 1 << 20, 0.01, 4),  # burn-in ignores the trial limit
+    (1 << 18, True, 1 << 20, 0.01, 4),   # ... and never checks convergence
+]
+
+
 @pytest.mark.parametrize("policy", ["bayes", "wald"])
-def test_probability_analysis_early_stop(gpu, policy):
+@pytest.mark.parametrize("num_trials,early_stop,burn_in,delta,want", EARLY_STOP_CASES)
+def test_probability_analysis_stopping_rules(gpu, policy, num_trials, early_stop, burn_in, delta,
+                                             want):
     pd, exact = core_models()["abc"]
-    settings = mc.Settings(num_trials=0, burn_in_trials=1 << 16, ci_rel_margin=0.01,
-                           ci_policy=policy)
-    p, tallies, passes = mc.probability_analysis(pd, settings, 1 << 12)
+    words = 1 << 12
+    settings = mc.Settings(num_trials=num_trials, burn_in_trials=burn_in, ci_rel_margin=delta,
+                           ci_policy=policy, early_stop=early_stop)
+    p, tallies, passes = mc.probability_analysis(pd, settings, words)
+    assert passes == want
     t = tallies[0]
+    assert t.total_bits == want * words * 64
+    # The same stream as the layer manager run for that many passes.
+    with mc.LayerManager(pd, words) as eng:
+        r = eng.pass_wait_collect(want)[0]
+    assert (p, t.num_one_bits) == (r.mean, r.num_one_bits)
     assert abs(p - exact) <= 3.2905 * t.std_err
-    # Replay the controller on the host over the same passes: burn-in to
-    # 2^16 trials, then one pass at a time until both targets have been met
-    # (src/mc/scheduler/convergence_controller.h:294-397, 471-482).
+
+
+def test_probability_analysis_early_stop_closed_form(gpu):
+    """Wald early stop on a model whose burn-in does not converge: the run
+    ends after the first post-burn-in pass at which both closed-form targets
+    hold, found by scanning the engine's cumulative tallies pass by pass."""
+    pd, _ = core_models()["atleast"]  # p = 0.098
+    words, delta = 1 << 12, 0.005
     z = mc.normal_quantile_two_sided(0.99)
-    eps_ok = log_ok = False
-    n = 0
-    with mc.LayerManager(pd, 1 << 12) as eng:
-        while n * (1 << 18) < (1 << 16):
+    settings = mc.Settings(num_trials=0, burn_in_trials=1 << 18, ci_rel_margin=delta,
+                           ci_policy="wald")
+    p, tallies, passes = mc.probability_analysis(pd, settings, words)
+    with mc.LayerManager(pd, words) as eng:
+        n = 1
+        eng.pass_wait_collect(1)  # burn-in: never checked
+        while True:
             r = eng.pass_wait_collect(1)[0]
             n += 1
-
-        def update(r):
-            c = mc.convergence_update(policy, r.num_one_bits, r.total_bits, 0.01, z)
-            ok = c["epsilon"] > 0 and c["epsilon"] <= 0.01 * max(r.mean, 1e-12)
-            return ok, c["epsilon_log10"] > 0 and c["epsilon_log10"] <= 0.01
-
-        a, b = update(r)
-        eps_ok, log_ok = eps_ok or a, log_ok or b
-        while not (eps_ok and log_ok):
-            r = eng.pass_wait_collect(1)[0]
-            n += 1
-            a, b = update(r)
-            eps_ok, log_ok = eps_ok or a, log_ok or b
+            if _closed_form_wald_met(r.num_one_bits, r.total_bits, delta, z):
+                break
+    # The linear target needs z^2 (1-p) / (p delta^2) = 2.4e6 trials, about
+    # nine passes of 2^18, so the stop falls well after burn-in.
+    assert n >= 6
     assert (passes, p) == (n, r.mean)
 
 

@@ -76,7 +76,7 @@ def test_csv_string_follows_the_reference_format():
 
 def test_row_has_the_reference_columns_in_order():
     pd = mc.small_tree(1)
-    s = mc.Settings(num_trials=1 << 22)
+    s = mc.Settings(num_trials=1 << 22, early_stop=False)
     row = report.convergence_row(pd, s, (16, 1024), _tally(790000, 1 << 22), 4, 1.25,
                                  input_file="small.xml")
     assert [k for k, _ in row] == SETTINGS + DEVICE + LAYERS + PDAG + TALLY + RUN
