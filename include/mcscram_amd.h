@@ -311,6 +311,29 @@ mcs_status mcs_probability_analysis(const mcs_pdag *pdag, const mcs_settings *se
                                     mcs_tally *out, int32_t n_out, double *p_total,
                                     uint32_t *passes_run);
 
+/*
+ * mcs_probability_analysis without any device allocation. No counterpart in
+ * the reference: it is for callers that cannot allocate device memory
+ * (serving loops and harnesses that forbid hipMalloc). The engine's device
+ * memory (words of every node, descriptors, argument rows, one-bit counters)
+ * lives in a fixed per-device array of MCS_NOALLOC_SCRATCH_BYTES, which each
+ * GPU the call runs on holds until the process ends; a PDAG and shape whose
+ * engine does not fit fail with MCS_ERR_OUT_OF_MEMORY and a message giving
+ * both sizes (one event buffer at 1 + events + gates rows of words per pass,
+ * 8 bytes a word, plus its side arrays). The call neither allocates nor frees
+ * device memory and creates no stream: it launches on `stream` (a
+ * hipStream_t of the current device; null = the null stream) and waits on
+ * it. Results equal mcs_probability_analysis's bit for bit. Every call on a
+ * GPU shares that GPU's array: calls that overlap on one GPU (other threads,
+ * other streams) race and must be serialised by the caller.
+ */
+#define MCS_NOALLOC_SCRATCH_BYTES (64u << 20)
+mcs_status mcs_probability_analysis_noalloc(const mcs_pdag *pdag, const mcs_settings *settings,
+                                            const mcs_sample_shape *shape,
+                                            const int32_t *observe, int32_t n_observe,
+                                            mcs_tally *out, int32_t n_out, double *p_total,
+                                            uint32_t *passes_run, void *stream);
+
 
 /*
  * Batches of independent trees evaluated by one persistent launch per pass.

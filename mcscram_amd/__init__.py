@@ -7,7 +7,8 @@ that needs it fails loudly if it was not built (python -m mcscram_amd.build).
 """
 from .pdag import (AND, ATLEAST, NAND, NOR, NOT, NULL, OR, XOR, Pdag,  # noqa: F401
                    batched_trees, large_tree, random_layered_tree, small_tree)
-from .engine import (Batch, LayerManager, Settings, Tally, beta_quantile,  # noqa: F401
+from .engine import (NOALLOC_SCRATCH_BYTES, Batch, LayerManager, Settings, Tally,  # noqa: F401
+                     beta_quantile,
                      convergence_update, library_path, load_library,
                      normal_quantile_two_sided, pdag_event_order, pdag_gate_depths,
                      plan_passes, probability_analysis, sample_shape_for)

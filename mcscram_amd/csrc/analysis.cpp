@@ -226,6 +226,17 @@ mcs_status mcs_probability_analysis(const mcs_pdag *pdag, const mcs_settings *se
                                     const mcs_sample_shape *shape, const int32_t *observe,
                                     int32_t n_observe, mcs_tally *out, int32_t n_out,
                                     double *p_total, uint32_t *passes_run) {
+    return mcs::probability_analysis(pdag, settings, shape, observe, n_observe, out, n_out,
+                                     p_total, passes_run, nullptr);
+}
+
+}  // extern "C"
+
+mcs_status mcs::probability_analysis(const mcs_pdag *pdag, const mcs_settings *settings,
+                                     const mcs_sample_shape *shape, const int32_t *observe,
+                                     int32_t n_observe, mcs_tally *out, int32_t n_out,
+                                     double *p_total, uint32_t *passes_run,
+                                     const ExternalMemory *ext) {
     if (!pdag || !settings || !shape || !out || !p_total || n_observe < 0 ||
         (n_observe > 0 && !observe)) {
         mcs::set_error("null argument");
@@ -239,8 +250,8 @@ mcs_status mcs_probability_analysis(const mcs_pdag *pdag, const mcs_settings *se
     tallies.push_back(pdag->root);
     for (int32_t i = 0; i < n_observe; ++i) tallies.push_back(observe[i]);
     mcs_engine *eng = nullptr;
-    mcs_status st = mcs_engine_create_ex(pdag, shape, tallies.data(), int32_t(tallies.size()),
-                                         settings->key0, settings->key1, -1, &eng);
+    mcs_status st = mcs::create_engine(pdag, shape, tallies.data(), int32_t(tallies.size()),
+                                       settings->key0, settings->key1, -1, ext, &eng);
     if (st != MCS_OK) return st;
     mcs_engine_info info{};
     mcs_engine_get_info(eng, &info);
@@ -315,5 +326,3 @@ mcs_status mcs_probability_analysis(const mcs_pdag *pdag, const mcs_settings *se
     mcs_engine_destroy(eng);
     return st;
 }
-
-}  // extern "C"

@@ -43,6 +43,8 @@ using mcs::fail;
 struct mcs_engine {
     int device = 0;
     hipStream_t stream = nullptr;
+    bool owns_memory = true;  // false: buffers carved from caller-provided memory
+    bool owns_stream = true;
     uint32_t key0 = MCS_REFERENCE_KEY0, key1 = MCS_REFERENCE_KEY1;
     uint32_t iteration = 0;  // passes submitted (reference iterable_queueable counter)
     uint32_t primed = 0;     // iteration already sampled into its buffer (0 = none)
@@ -80,17 +82,19 @@ struct mcs_engine {
     uint32_t n_launches() const { return uint32_t(launch_gate_begin.size() - 1); }
 
     ~mcs_engine() {
-        if (stream) {
+        if (stream || !owns_stream) {
             (void)hipSetDevice(device);
             (void)hipStreamSynchronize(stream);
         }
-        if (d_words) (void)hipFree(d_words);
-        if (d_events) (void)hipFree(d_events);
-        if (d_gates) (void)hipFree(d_gates);
-        if (d_probe) (void)hipFree(d_probe);
-        if (d_args) (void)hipFree(d_args);
-        if (d_ones) (void)hipFree(d_ones);
-        if (stream) (void)hipStreamDestroy(stream);
+        if (owns_memory) {
+            if (d_words) (void)hipFree(d_words);
+            if (d_events) (void)hipFree(d_events);
+            if (d_gates) (void)hipFree(d_gates);
+            if (d_probe) (void)hipFree(d_probe);
+            if (d_args) (void)hipFree(d_args);
+            if (d_ones) (void)hipFree(d_ones);
+        }
+        if (stream && owns_stream) (void)hipStreamDestroy(stream);
     }
 };
 
@@ -115,9 +119,23 @@ uint32_t pick_chunk_words(uint64_t n_bp) {
     return 512;
 }
 
+// Bump allocator over caller-provided memory: 256-byte aligned pieces.
+struct Carver {
+    uint8_t *base;
+    uint64_t bytes, used = 0;
+    template <class T>
+    T *take(uint64_t n) {
+        used = (used + 255) & ~uint64_t(255);
+        T *p = base ? reinterpret_cast<T *>(base + used) : nullptr;
+        used += std::max<uint64_t>(n, 1) * sizeof(T);
+        return p;
+    }
+};
+
 mcs_status build_engine(const mcs_pdag *pdag, const mcs_sample_shape *shape,
                         const int32_t *tally_index, int32_t n_tally, uint32_t key0,
-                        uint32_t key1, int32_t device, mcs_engine **out) {
+                        uint32_t key1, int32_t device, const mcs::ExternalMemory *ext,
+                        mcs_engine **out) {
     if (!pdag || !shape || !out || (n_tally > 0 && !tally_index) || n_tally < 0)
         return fail(MCS_ERR_INVALID_ARGUMENT, "null argument");
     *out = nullptr;
@@ -161,9 +179,13 @@ mcs_status build_engine(const mcs_pdag *pdag, const mcs_sample_shape *shape,
     if (device < 0) MCS_HIP(hipGetDevice(&eng->device));
     MCS_HIP(hipSetDevice(eng->device));
     size_t free_b = 0, total_b = 0;
-    MCS_HIP(hipMemGetInfo(&free_b, &total_b));
+    if (ext) {
+        free_b = size_t(ext->bytes);  // the whole region, nothing held back
+    } else {
+        MCS_HIP(hipMemGetInfo(&free_b, &total_b));
+    }
     const uint64_t row_bytes = geo.stride * sizeof(uint64_t);
-    const uint64_t reserve = uint64_t(256) << 20;
+    const uint64_t reserve = ext ? 0 : uint64_t(256) << 20;
     const uint64_t one_buf = (uint64_t(plan.n_nodes()) * row_bytes);
     const uint64_t two_buf = one_buf + uint64_t(n_ev) * row_bytes;
     // Execution schedule (wide gates split into partial folds, each partial
@@ -172,14 +194,31 @@ mcs_status build_engine(const mcs_pdag *pdag, const mcs_sample_shape *shape,
     mcs::fuse_single_consumer(sch, plan, kTaskRows);
     const uint64_t aux_bytes = uint64_t(sch.n_aux) * row_bytes;
     eng->n_buf = (n_ev > 0 && two_buf + aux_bytes + reserve <= free_b) ? 2u : 1u;
-    eng->n_rows = uint64_t(eng->n_buf) * n_ev + plan.n_gates() + uint64_t(sch.n_aux);
 
     // Launch plan: one launch per schedule layer; launch 0 also carries the
-    // tally-only tasks of tallied basic events (slot = event id).
+    // tally-only tasks of tallied basic events (slot = event id). With
+    // caller memory the second event buffer is kept only if every carved
+    // array still fits; the plan's row numbers depend on the buffer count,
+    // so it is rebuilt for one buffer otherwise.
     mcs::TreeLaunchPlan lp;
-    const mcs::RowMap rows{{0u, eng->n_buf == 2 ? uint32_t(n_ev) : 0u},
+    mcs::RowMap rows{};
+    for (;;) {
+        eng->n_rows = uint64_t(eng->n_buf) * n_ev + plan.n_gates() + uint64_t(sch.n_aux);
+        rows = mcs::RowMap{{0u, eng->n_buf == 2 ? uint32_t(n_ev) : 0u},
                            uint32_t(eng->n_buf) * uint32_t(n_ev), 0u};
-    mcs::build_launch_plan(plan, sch, tallied, rows, lp);
+        lp = mcs::TreeLaunchPlan{};
+        mcs::build_launch_plan(plan, sch, tallied, rows, lp);
+        if (!ext || eng->n_buf == 1) break;
+        Carver c{nullptr, ext->bytes};
+        c.take<uint64_t>(eng->n_rows * geo.stride);
+        c.take<unsigned long long>(eng->n_rows);
+        c.take<mcs::EventDesc>(uint64_t(n_ev));
+        c.take<mcs::GateDesc>(lp.gates.size());
+        c.take<mcs::GateDesc>(1);
+        c.take<uint32_t>(std::max<uint64_t>(lp.args[0].size(), 1) * eng->n_buf);
+        if (c.used <= c.bytes) break;
+        eng->n_buf = 1;
+    }
     const std::vector<mcs::GateDesc> &gd = lp.gates;
     const std::vector<uint32_t> &args0 = lp.args[0], &args1 = lp.args[1];
     eng->launch_gate_begin = lp.launch_begin;
@@ -219,23 +258,44 @@ mcs_status build_engine(const mcs_pdag *pdag, const mcs_sample_shape *shape,
     for (int32_t e = 0; e < n_ev; ++e)
         ev[e] = mcs::make_event_desc(plan.event_pdag_index[e], uint32_t(e), plan.event_p[e]);
 
-    // Device allocation.
-    MCS_HIP(hipStreamCreateWithFlags(&eng->stream, hipStreamNonBlocking));
+    // Device memory: allocated, or carved from the caller's region.
     const uint64_t word_bytes = eng->n_rows * row_bytes;
     const uint64_t need = word_bytes + ev.size() * sizeof(mcs::EventDesc) +
                           gd.size() * sizeof(mcs::GateDesc) + eng->arg_span * 4 * eng->n_buf +
                           eng->n_rows * 8;
-    if (need + reserve > free_b)
-        return fail(MCS_ERR_OUT_OF_MEMORY,
-                    "engine needs " + std::to_string(need >> 20) + " MiB of device memory, " +
-                        std::to_string(free_b >> 20) + " MiB free: reduce words per pass");
-    MCS_HIP(hipMalloc(&eng->d_words, word_bytes));
-    MCS_HIP(hipMalloc(&eng->d_ones, eng->n_rows * sizeof(unsigned long long)));
-    if (!ev.empty()) MCS_HIP(hipMalloc(&eng->d_events, ev.size() * sizeof(mcs::EventDesc)));
-    MCS_HIP(hipMalloc(&eng->d_gates, std::max<size_t>(gd.size(), 1) * sizeof(mcs::GateDesc)));
-    MCS_HIP(hipMalloc(&eng->d_probe, sizeof(mcs::GateDesc)));
-    MCS_HIP(hipMalloc(&eng->d_args, eng->arg_span * 4 * eng->n_buf));
-    eng->device_bytes = need;
+    if (ext) {
+        // Not the engine's to free, even if the carve below does not fit.
+        eng->owns_memory = false;
+        eng->owns_stream = false;
+        eng->stream = ext->stream;
+        Carver c{static_cast<uint8_t *>(ext->base), ext->bytes};
+        eng->d_words = c.take<uint64_t>(word_bytes / 8);
+        eng->d_ones = c.take<unsigned long long>(eng->n_rows);
+        eng->d_events = ev.empty() ? nullptr : c.take<mcs::EventDesc>(ev.size());
+        eng->d_gates = c.take<mcs::GateDesc>(gd.size());
+        eng->d_probe = c.take<mcs::GateDesc>(1);
+        eng->d_args = c.take<uint32_t>(eng->arg_span * eng->n_buf);
+        if (c.used > c.bytes)
+            return fail(MCS_ERR_OUT_OF_MEMORY,
+                        "engine needs " + std::to_string(c.used) + " bytes of device memory, " +
+                            std::string(ext->name ? ext->name : "the caller's region") +
+                            " holds " + std::to_string(c.bytes) +
+                            ": reduce words per pass or the tree");
+        eng->device_bytes = c.used;
+    } else {
+        if (need + reserve > free_b)
+            return fail(MCS_ERR_OUT_OF_MEMORY,
+                        "engine needs " + std::to_string(need >> 20) + " MiB of device memory, " +
+                            std::to_string(free_b >> 20) + " MiB free: reduce words per pass");
+        MCS_HIP(hipStreamCreateWithFlags(&eng->stream, hipStreamNonBlocking));
+        MCS_HIP(hipMalloc(&eng->d_words, word_bytes));
+        MCS_HIP(hipMalloc(&eng->d_ones, eng->n_rows * sizeof(unsigned long long)));
+        if (!ev.empty()) MCS_HIP(hipMalloc(&eng->d_events, ev.size() * sizeof(mcs::EventDesc)));
+        MCS_HIP(hipMalloc(&eng->d_gates, std::max<size_t>(gd.size(), 1) * sizeof(mcs::GateDesc)));
+        MCS_HIP(hipMalloc(&eng->d_probe, sizeof(mcs::GateDesc)));
+        MCS_HIP(hipMalloc(&eng->d_args, eng->arg_span * 4 * eng->n_buf));
+        eng->device_bytes = need;
+    }
     if (!ev.empty())
         MCS_HIP(hipMemcpyAsync(eng->d_events, ev.data(), ev.size() * sizeof(mcs::EventDesc),
                                hipMemcpyHostToDevice, eng->stream));
@@ -373,6 +433,13 @@ struct EventTimer : LaunchHook {
 
 }  // namespace
 
+mcs_status mcs::create_engine(const mcs_pdag *pdag, const mcs_sample_shape *shape,
+                              const int32_t *tally_index, int32_t n_tally, uint32_t key0,
+                              uint32_t key1, int32_t device, const ExternalMemory *ext,
+                              mcs_engine **out) {
+    return build_engine(pdag, shape, tally_index, n_tally, key0, key1, device, ext, out);
+}
+
 extern "C" {
 
 const char *mcs_last_error(void) { return g_error.c_str(); }
@@ -384,13 +451,13 @@ void mcs_tally_point_estimates(mcs_tally *t) {
 mcs_status mcs_engine_create(const mcs_pdag *pdag, const mcs_sample_shape *shape,
                              const int32_t *tally_index, int32_t n_tally, mcs_engine **out) {
     return build_engine(pdag, shape, tally_index, n_tally, MCS_REFERENCE_KEY0,
-                        MCS_REFERENCE_KEY1, -1, out);
+                        MCS_REFERENCE_KEY1, -1, nullptr, out);
 }
 
 mcs_status mcs_engine_create_ex(const mcs_pdag *pdag, const mcs_sample_shape *shape,
                                 const int32_t *tally_index, int32_t n_tally, uint32_t key0,
                                 uint32_t key1, int32_t device, mcs_engine **out) {
-    return build_engine(pdag, shape, tally_index, n_tally, key0, key1, device, out);
+    return build_engine(pdag, shape, tally_index, n_tally, key0, key1, device, nullptr, out);
 }
 
 mcs_status mcs_engine_destroy(mcs_engine *engine) {

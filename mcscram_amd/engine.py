@@ -119,6 +119,7 @@ def load_library():
         "mcs_pdag_event_order": [vp, vp, i32, vp],
         "mcs_pdag_gate_depths": [vp, vp, i32],
         "mcs_probability_analysis": [vp, vp, vp, vp, i32, vp, i32, vp, vp],
+        "mcs_probability_analysis_noalloc": [vp, vp, vp, vp, i32, vp, i32, vp, vp, vp],
         "mcs_batch_create": [vp, i32, vp, vp, vp, vp, vp],
         "mcs_batch_destroy": [vp],
         "mcs_batch_pass": [vp, u32],
@@ -319,8 +320,13 @@ class Settings:
                          CI_POLICIES[self.ci_policy], int(bool(self.early_stop)))
 
 
-def probability_analysis(pdag: Pdag, settings: Settings, shape, observe=()):
-    """-> (p_total, [root tally, observed tallies...], passes run)."""
+def probability_analysis(pdag: Pdag, settings: Settings, shape, observe=(), *,
+                         no_alloc: bool = False, stream: int | None = None):
+    """-> (p_total, [root tally, observed tallies...], passes run).
+
+    no_alloc runs mcs_probability_analysis_noalloc: the engine lives in the
+    library's fixed per-device scratch (NOALLOC_SCRATCH_BYTES) and launches
+    on ``stream`` (a hipStream_t as an int; None = the null stream)."""
     L = load_library()
     view = _PdagView(pdag)
     sh = _shape(shape)
@@ -330,10 +336,17 @@ def probability_analysis(pdag: Pdag, settings: Settings, shape, observe=()):
     p = C.c_double()
     passes = C.c_uint32()
     st = settings._c()
-    _check(L.mcs_probability_analysis(C.byref(view.struct), C.byref(st), C.byref(sh), _p(obs),
-                                      n_obs, out, 1 + n_obs, C.byref(p), C.byref(passes)))
+    args = (C.byref(view.struct), C.byref(st), C.byref(sh), _p(obs), n_obs, out, 1 + n_obs,
+            C.byref(p), C.byref(passes))
+    if no_alloc:
+        _check(L.mcs_probability_analysis_noalloc(*args, C.c_void_p(stream or 0)))
+    else:
+        _check(L.mcs_probability_analysis(*args))
     distinct = 1 + len(set(observe) - {pdag.root})
     return p.value, [Tally._from(t) for t in out[:distinct]], passes.value
+
+
+NOALLOC_SCRATCH_BYTES = 64 << 20  # MCS_NOALLOC_SCRATCH_BYTES
 
 
 def plan_passes(num_trials: int, shape) -> tuple[int, int]:
