@@ -70,6 +70,8 @@ struct mcs_engine {
     mcs::GateDesc probe_host{};             // its host copy (source of the async upload)
     uint32_t *d_args = nullptr;             // n_buf arrays of arg_span entries
     unsigned long long *d_ones = nullptr;
+    uint8_t *d_state = nullptr;             // chunk summaries [n_rows][state_stride]
+    uint32_t state_stride = 0;
 
     uint64_t gate_row(int32_t node) const {
         return uint64_t(n_buf) * plan.n_events() + uint64_t(node - plan.n_events());
@@ -93,6 +95,7 @@ struct mcs_engine {
             if (d_probe) (void)hipFree(d_probe);
             if (d_args) (void)hipFree(d_args);
             if (d_ones) (void)hipFree(d_ones);
+            if (d_state) (void)hipFree(d_state);
         }
         if (stream && owns_stream) (void)hipStreamDestroy(stream);
     }
@@ -216,6 +219,7 @@ mcs_status build_engine(const mcs_pdag *pdag, const mcs_sample_shape *shape,
         c.take<mcs::GateDesc>(lp.gates.size());
         c.take<mcs::GateDesc>(1);
         c.take<uint32_t>(std::max<uint64_t>(lp.args[0].size(), 1) * eng->n_buf);
+        c.take<uint8_t>(eng->n_rows * mcs::state_stride_for(geo.n_chunks));
         if (c.used <= c.bytes) break;
         eng->n_buf = 1;
     }
@@ -259,10 +263,12 @@ mcs_status build_engine(const mcs_pdag *pdag, const mcs_sample_shape *shape,
         ev[e] = mcs::make_event_desc(plan.event_pdag_index[e], uint32_t(e), plan.event_p[e]);
 
     // Device memory: allocated, or carved from the caller's region.
+    eng->state_stride = mcs::state_stride_for(geo.n_chunks);
+    const uint64_t state_bytes = eng->n_rows * eng->state_stride;
     const uint64_t word_bytes = eng->n_rows * row_bytes;
     const uint64_t need = word_bytes + ev.size() * sizeof(mcs::EventDesc) +
                           gd.size() * sizeof(mcs::GateDesc) + eng->arg_span * 4 * eng->n_buf +
-                          eng->n_rows * 8;
+                          eng->n_rows * 8 + state_bytes;
     if (ext) {
         // Not the engine's to free, even if the carve below does not fit.
         eng->owns_memory = false;
@@ -275,6 +281,7 @@ mcs_status build_engine(const mcs_pdag *pdag, const mcs_sample_shape *shape,
         eng->d_gates = c.take<mcs::GateDesc>(gd.size());
         eng->d_probe = c.take<mcs::GateDesc>(1);
         eng->d_args = c.take<uint32_t>(eng->arg_span * eng->n_buf);
+        eng->d_state = c.take<uint8_t>(state_bytes);
         if (c.used > c.bytes)
             return fail(MCS_ERR_OUT_OF_MEMORY,
                         "engine needs " + std::to_string(c.used) + " bytes of device memory, " +
@@ -294,6 +301,7 @@ mcs_status build_engine(const mcs_pdag *pdag, const mcs_sample_shape *shape,
         MCS_HIP(hipMalloc(&eng->d_gates, std::max<size_t>(gd.size(), 1) * sizeof(mcs::GateDesc)));
         MCS_HIP(hipMalloc(&eng->d_probe, sizeof(mcs::GateDesc)));
         MCS_HIP(hipMalloc(&eng->d_args, eng->arg_span * 4 * eng->n_buf));
+        MCS_HIP(hipMalloc(&eng->d_state, std::max<uint64_t>(state_bytes, 1)));
         eng->device_bytes = need;
     }
     if (!ev.empty())
@@ -310,6 +318,8 @@ mcs_status build_engine(const mcs_pdag *pdag, const mcs_sample_shape *shape,
                                    hipMemcpyHostToDevice, eng->stream));
     }
     MCS_HIP(hipMemsetAsync(eng->d_ones, 0, eng->n_rows * sizeof(unsigned long long), eng->stream));
+    // A summary no task has written yet says "mixed": its words are loaded.
+    MCS_HIP(hipMemsetAsync(eng->d_state, mcs::kChunkMixed, state_bytes, eng->stream));
     MCS_HIP(hipStreamSynchronize(eng->stream));
     eng->key0 = key0;
     eng->key1 = key1;
@@ -322,6 +332,8 @@ mcs::PassLaunch base_launch(const mcs_engine *e) {
     pl.events = e->d_events;
     pl.key0 = e->key0;
     pl.key1 = e->key1;
+    pl.state = e->d_state;
+    pl.state_stride = e->state_stride;
     return pl;
 }
 
@@ -547,6 +559,7 @@ mcs_status mcs_engine_read_words(mcs_engine *engine, int32_t pdag_index, uint64_
         mcs::GateDesc &d = engine->probe_host;
         d = engine->fused_desc[size_t(node)];
         d.type_k_flags &= ~(mcs::kGateNoStore | mcs::kGateTallied);
+        d.type_k_flags |= mcs::kGateForceStore;
         MCS_HIP(hipMemcpyAsync(engine->d_probe, &d, sizeof(d), hipMemcpyHostToDevice,
                                engine->stream));
         mcs::PassLaunch pl = base_launch(engine);
@@ -558,7 +571,21 @@ mcs_status mcs_engine_read_words(mcs_engine *engine, int32_t pdag_index, uint64_
     }
     MCS_HIP(hipMemcpyAsync(out, engine->d_words + row * engine->geo.stride, engine->geo.n_bp * 8,
                            hipMemcpyDeviceToHost, engine->stream));
+    std::vector<uint8_t> summary(engine->geo.n_chunks);
+    MCS_HIP(hipMemcpyAsync(summary.data(), engine->d_state + row * engine->state_stride,
+                           summary.size(), hipMemcpyDeviceToHost, engine->stream));
     MCS_HIP(hipStreamSynchronize(engine->stream));
+    // Chunks whose summary is constant were not stored: their words are the
+    // constant (fused gates were just stored whole).
+    if (!engine->fused[size_t(node)]) {
+        const uint64_t cw = engine->geo.chunk_words;
+        for (uint32_t c = 0; c < engine->geo.n_chunks; ++c) {
+            if (summary[c] == mcs::kChunkMixed) continue;
+            const uint64_t w = summary[c] == mcs::kChunkOnes ? ~uint64_t(0) : 0;
+            const uint64_t end = std::min<uint64_t>((c + 1) * cw, engine->geo.n_bp);
+            std::fill(out + c * cw, out + end, w);
+        }
+    }
     return MCS_OK;
 }
 

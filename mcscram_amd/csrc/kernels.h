@@ -46,8 +46,19 @@ inline EventDesc make_event_desc(int32_t pdag_index, uint32_t event_idx, double 
 }
 
 // Gate flags (GateDesc::type_k_flags bits 16..).
-constexpr uint32_t kGateTallied = 1u << 16;  // popcount into ones[out_row]
-constexpr uint32_t kGateNoStore = 1u << 17;  // tally only (basic-event tallies)
+constexpr uint32_t kGateTallied = 1u << 16;     // popcount into ones[out_row]
+constexpr uint32_t kGateNoStore = 1u << 17;     // tally only (basic-event tallies)
+constexpr uint32_t kGateForceStore = 1u << 18;  // store the words even when constant
+
+// Chunk summaries: one byte per (row, chunk) says whether every live word of
+// the row's chunk is zero, every one is all ones, or neither. Every task
+// that produces a chunk writes its summary; a gate task reads its inputs'
+// summaries first, so a constant input is never loaded (it is folded in as
+// a constant, or decides the gate outright: AND with a zero chunk), and a
+// gate chunk whose summary is constant is not stored at all. Rows of one
+// pass are summarised in that pass, so a summary always describes the words
+// (stored or implied) of the pass that reads it.
+enum : uint32_t { kChunkZero = 0, kChunkOnes = 1, kChunkMixed = 2 };
 
 // An argument entry with this bit set is not a row: the low bits index the
 // descriptor of a fused gate (in PassLaunch::tree_gates) whose word the task
@@ -82,7 +93,13 @@ struct PassLaunch {
     uint64_t samp_begin, samp_end;
     uint32_t samp_row0;
     uint32_t key0, key1;
+    uint8_t *state;                // chunk summaries [row][state_stride] (null: none,
+                                   // every chunk is stored and loaded)
+    uint32_t state_stride;         // bytes per row: n_chunks rounded up to 4
 };
+
+// Bytes per row of the chunk-summary array.
+inline uint32_t state_stride_for(uint32_t n_chunks) { return (n_chunks + 3u) & ~3u; }
 
 // Workgroups of a pass_layer launch are dealt to the XCDs round-robin:
 // workgroup b runs on XCD b % kXcds as that XCD's (b / kXcds)-th workgroup.

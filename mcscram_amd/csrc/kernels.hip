@@ -54,6 +54,38 @@ __device__ __forceinline__ void tally_wave(unsigned long long *ones, uint32_t sl
     if ((threadIdx.x & 63u) == 0 && pc) atomicAdd(ones + slot, (unsigned long long)pc);
 }
 
+// Scalar loads: argument rows and chunk summaries are written before the
+// launch that reads them (summaries of a launch's own outputs are never read
+// by it), so they are read through the constant address space, which keeps
+// the loads on the scalar unit.
+typedef const __attribute__((address_space(4))) uint32_t const_u32;
+
+__device__ __forceinline__ uint32_t sload(const uint32_t *p, uint64_t i) {
+    return ((const_u32 *)p)[i];
+}
+
+// Workgroup-wide OR of two flags (any live 1 bit, any live 0 bit) through
+// LDS: every lane of the workgroup calls it with its own flags and gets the
+// workgroup's summary of the chunk.
+__device__ __forceinline__ uint32_t chunk_summary(bool any1, bool any0) {
+    __shared__ uint32_t flags[kLanes / 64];
+    const uint32_t w = (__builtin_amdgcn_ballot_w64(any1) != 0 ? 1u : 0u) |
+                       (__builtin_amdgcn_ballot_w64(any0) != 0 ? 2u : 0u);
+    if ((threadIdx.x & 63u) == 0) flags[threadIdx.x >> 6] = w;
+    __syncthreads();
+    uint32_t f = 0;
+#pragma unroll
+    for (uint32_t i = 0; i < kLanes / 64; ++i) f |= flags[i];
+    __syncthreads();  // flags[] is reused by the workgroup's next task
+    // any 1 and any 0: mixed; only 1s: all ones; otherwise all zero.
+    return f == 3u ? kChunkMixed : (f == 1u ? kChunkOnes : kChunkZero);
+}
+
+__device__ __forceinline__ void put_summary(const PassLaunch &L, uint32_t row, uint32_t chunk,
+                                            uint32_t st) {
+    if (threadIdx.x == 0) L.state[uint64_t(row) * L.state_stride + chunk] = uint8_t(st);
+}
+
 // Sampler task: words [base, base + chunk_words) of one event.
 __device__ __forceinline__ void sampler_task(const PassLaunch &L, uint64_t *__restrict__ words,
                                              const LaunchGeometry &geo, const SplitFracs &sf,
@@ -66,6 +98,7 @@ __device__ __forceinline__ void sampler_task(const PassLaunch &L, uint64_t *__re
     const uint32_t c3base = (iteration + 1u) << 6;
     uint64_t *row = words + uint64_t(L.samp_row0 + e) * geo.stride;
     const uint64_t base = uint64_t(chunk) * geo.chunk_words;
+    uint64_t any = 0, all = ~0ull;  // over the lane's live words
     for (uint32_t off = threadIdx.x * 2u; off < geo.chunk_words; off += 2u * kLanes) {
         const uint64_t s = base + off;
         uint64_t w0, w1;
@@ -76,6 +109,14 @@ __device__ __forceinline__ void sampler_task(const PassLaunch &L, uint64_t *__re
             w0 = w1 = (d.mode == kEventAlways) ? ~0ull : 0ull;
         }
         store_pair(row + s, w0, w1);
+        if (L.state) {
+            if (s < geo.n_bp) { any |= w0; all &= w0; }
+            if (s + 1 < geo.n_bp) { any |= w1; all &= w1; }
+        }
+    }
+    if (L.state) {
+        const uint32_t st = chunk_summary(any != 0, all != ~0ull);
+        put_summary(L, L.samp_row0 + e, chunk, st);
     }
 }
 
@@ -100,6 +141,8 @@ struct Task {
     uint32_t lane_off;    // 16 * lane
     uint32_t chunk_bytes; // 8 * chunk words
     bool full;            // every word of the chunk is below n_bp (no padding)
+    uint32_t chunk;       // chunk index (summary column)
+    uint32_t live;        // live words of the chunk
 };
 
 constexpr int32_t kBufferWord3 = 0x00020000;  // raw buffer, 32-bit format fields (gfx9)
@@ -127,6 +170,97 @@ __device__ __forceinline__ uint64_t lane_word(const Task &t, int k, int h) {
 __device__ __forceinline__ uint32_t pair_popcount(const Task &t, const Pair &p, int k) {
     if (t.full) return uint32_t(__popcll(p.x)) + uint32_t(__popcll(p.y));
     return live_popcount(p.x, p.y, lane_word(t, k, 0), t.n_bp);
+}
+
+// Summary of the task's chunk of a row (kChunkMixed without summaries).
+__device__ __forceinline__ uint32_t row_summary(const Task &t, uint32_t row) {
+    if (!t.L->state) return kChunkMixed;
+    const uint64_t b = uint64_t(row) * t.L->state_stride + t.chunk;
+    const uint32_t w = sload(reinterpret_cast<const uint32_t *>(t.L->state), b >> 2);
+    return (w >> (8u * uint32_t(b & 3u))) & 0xFFu;
+}
+
+// Classification of a gate over the task's chunk from its inputs'
+// summaries: constant (every live word 0 or all ones, without loading an
+// input), or the work left: the inputs still to fold (`mixed`), the XOR
+// parity of the constant inputs, and ATLEAST's threshold over the mixed
+// inputs (the constant true inputs already counted). An ATLEAST with 256 or
+// more inputs keeps every input, since its 8-bit count wraps.
+struct Cls {
+    int32_t konst;   // -1: evaluate; 0 or 1: the gate's value (after NOT/NAND/NOR)
+    uint32_t k8;     // ATLEAST threshold over the mixed inputs
+    uint32_t flip;   // XOR: parity of the constant true inputs
+    uint32_t mixed;  // inputs left to fold
+};
+
+// Value of one input over the chunk before the gate folds it: 0, 1 or
+// kChunkMixed. With kFusedArgs an input may be a fused gate, whose value
+// comes from its own inputs' summaries (fused gates never hold fused gates,
+// so the classification has two levels).
+template <bool kFusedArgs>
+__device__ __forceinline__ uint32_t arg_value(uint32_t ai, bool neg, const Task &t);
+
+template <bool kFusedArgs>
+__device__ __forceinline__ Cls classify(uint32_t type, uint32_t k8, uint32_t n_in, uint32_t n_pos,
+                                        const uint32_t *__restrict__ a, const Task &t) {
+    Cls c{-1, k8, 0, n_in};
+    if (!t.L->state || (type == 2u && n_in >= 256u)) return c;
+    uint32_t n0 = 0, n1 = 0, last = kChunkZero;
+    for (uint32_t i = 0; i < n_in; ++i) {
+        last = arg_value<kFusedArgs>(sload(a, i), i >= n_pos, t);
+        n0 += last == 0u;
+        n1 += last == 1u;
+    }
+    const uint32_t m = n_in - n0 - n1;
+    c.mixed = m;
+    int32_t v = -1;
+    if (type == 0u || type == 5u) {         // AND, NAND
+        v = n0 ? 0 : (m == 0 ? 1 : -1);
+    } else if (type == 1u || type == 6u) {  // OR, NOR
+        v = n1 ? 1 : (m == 0 ? 0 : -1);
+    } else if (type == 3u) {                // XOR
+        c.flip = n1 & 1u;
+        v = m == 0 ? int32_t(c.flip) : -1;
+    } else if (type == 4u || type == 7u) {  // NOT, NULL: the last input
+        v = n_in == 0 ? 0 : (last == kChunkMixed ? -1 : int32_t(last));
+        c.mixed = v < 0 ? 1u : 0u;
+    } else {                                // ATLEAST, fewer than 256 inputs
+        const int32_t kk = int32_t(k8) - int32_t(n1);
+        v = kk <= 0 ? 1 : (kk > int32_t(m) ? 0 : -1);
+        c.k8 = kk > 0 ? uint32_t(kk) : 0u;
+    }
+    if (v >= 0 && (type == 4u || type == 5u || type == 6u)) v ^= 1;
+    c.konst = v;
+    return c;
+}
+
+template <bool kFusedArgs>
+__device__ __forceinline__ uint32_t arg_value(uint32_t ai, bool neg, const Task &t) {
+    uint32_t v;
+    if (kFusedArgs && (ai & kInlineArg)) {
+        const GateDesc c = t.L->tree_gates[ai & ~kInlineArg];
+        const Cls f = classify<false>(c.type_k_flags & 0xFFu, (c.type_k_flags >> 8) & 0xFFu,
+                                      c.counts & 0xFFFFu, c.counts >> 16,
+                                      t.L->arg_rows + c.arg_begin, t);
+        v = f.konst < 0 ? kChunkMixed : uint32_t(f.konst);
+    } else {
+        v = row_summary(t, ai);
+    }
+    return v == kChunkMixed ? v : (v ^ uint32_t(neg));
+}
+
+// Whether input ai (a row or a fused gate) is folded: false when its chunk
+// is constant (a constant the classification has already accounted for).
+template <bool kFusedArgs>
+__device__ __forceinline__ bool folded(uint32_t ai, const Task &t) {
+    return !t.L->state || arg_value<kFusedArgs>(ai, false, t) == kChunkMixed;
+}
+
+// The tally of a constant chunk: live words x 64 one bits, or none; added
+// once per workgroup.
+__device__ __forceinline__ void tally_constant(const Task &t, uint32_t slot, int32_t v) {
+    if (threadIdx.x == 0 && v == 1)
+        atomicAdd(t.ones + slot, (unsigned long long)t.live * 64ull);
 }
 
 // Truth table of v_bitop3_b32 for acc <op> v0' <op> v1', v' = v or ~v; the
@@ -188,19 +322,31 @@ __device__ __forceinline__ void fold_simple(Pair (&r)[K], const uint32_t *__rest
                                             uint32_t n_in, uint32_t n_pos, const Task &t,
                                             int k0 = 0);
 
+template <int K>
+__device__ __forceinline__ void fill(Pair (&v)[K], uint64_t w) {
+#pragma unroll
+    for (int k = 0; k < K; ++k) v[k] = {w, w};
+}
+
 // Word pairs k0 .. k0+K-1 of a fused gate (AND/OR/XOR/NAND/NOR/NOT/NULL over
-// rows), tallied: the popcount is added to the gate's counter here.
+// rows), tallied: the popcount is added to the gate's counter here. With
+// tally_constant_too false, a constant fused gate is left to the caller,
+// which has counted it already.
 template <int K>
 __device__ __forceinline__ void eval_fused(uint32_t ref, const Task &t, Pair (&v)[K], int k0 = 0) {
     const GateDesc c = t.L->tree_gates[ref & ~kInlineArg];
     const uint32_t *ca = t.L->arg_rows + c.arg_begin;
     const uint32_t n_in = c.counts & 0xFFFFu, n_pos = c.counts >> 16;
     const uint32_t type = c.type_k_flags & 0xFFu;
+    const Cls cls = classify<false>(type, 0u, n_in, n_pos, ca, t);
+    if (cls.konst >= 0) {  // its tally was counted where the constant was found
+        fill<K>(v, cls.konst ? ~0ull : 0ull);
+        return;
+    }
     if (type == 4u || type == 7u) {  // NOT, NULL: the last input
-#pragma unroll
-        for (int k = 0; k < K; ++k) v[k] = {0, 0};
+        fill<K>(v, 0ull);
         if (n_in) {
-            const __amdgpu_buffer_rsrc_t rp = chunk_of(t, uniform(ca[n_in - 1]));
+            const __amdgpu_buffer_rsrc_t rp = chunk_of(t, uniform(sload(ca, n_in - 1)));
 #pragma unroll
             for (int k = 0; k < K; ++k) {
                 v[k] = load_k(rp, t, k0 + k);
@@ -208,16 +354,13 @@ __device__ __forceinline__ void eval_fused(uint32_t ref, const Task &t, Pair (&v
             }
         }
     } else if (type == 0u || type == 5u) {
-#pragma unroll
-        for (int k = 0; k < K; ++k) v[k] = {~0ull, ~0ull};
+        fill<K>(v, ~0ull);
         fold_simple<K, 0, false>(v, ca, n_in, n_pos, t, k0);
     } else if (type == 3u) {
-#pragma unroll
-        for (int k = 0; k < K; ++k) v[k] = {0, 0};
+        fill<K>(v, cls.flip ? ~0ull : 0ull);
         fold_simple<K, 2, false>(v, ca, n_in, n_pos, t, k0);
     } else {
-#pragma unroll
-        for (int k = 0; k < K; ++k) v[k] = {0, 0};
+        fill<K>(v, 0ull);
         fold_simple<K, 1, false>(v, ca, n_in, n_pos, t, k0);
     }
     const bool invert = type == 4u || type == 5u || type == 6u;  // NOT, NAND, NOR
@@ -230,18 +373,37 @@ __device__ __forceinline__ void eval_fused(uint32_t ref, const Task &t, Pair (&v
     if (c.type_k_flags & kGateTallied) tally_wave(t.ones, c.out_row, pc);
 }
 
+// The tallies of the fused gates among a task's inputs whose chunks are
+// constant: their words are never computed, so their counts are added here,
+// once per workgroup. Called once per task.
+__device__ __forceinline__ void tally_constant_fused(const uint32_t *__restrict__ a, uint32_t n_in,
+                                                     const Task &t) {
+    if (!t.L->state) return;
+    for (uint32_t i = 0; i < n_in; ++i) {
+        const uint32_t ai = sload(a, i);
+        if (!(ai & kInlineArg)) continue;
+        const GateDesc c = t.L->tree_gates[ai & ~kInlineArg];
+        if (!(c.type_k_flags & kGateTallied)) continue;
+        const uint32_t v = arg_value<true>(ai, false, t);
+        if (v != kChunkMixed) tally_constant(t, c.out_row, int32_t(v));
+    }
+}
+
 // AND / OR / XOR families over K pairs per lane, row inputs two at a time
 // (2K 16-byte loads in flight per lane, one bitop3 per 32 bits for both).
-// With kFused an argument may reference a fused gate (kInlineArg), whose
-// word the task computes in registers; fused gates never reference fused
-// gates.
+// Inputs whose chunks are constant are skipped: the classification found
+// none that decides the gate, so each is neutral (AND: true, OR: false) or,
+// for XOR, part of the parity the caller starts from. With kFused an
+// argument may reference a fused gate (kInlineArg), whose word the task
+// computes in registers; fused gates never reference fused gates.
 template <int K, int Op, bool kFused>
 __device__ __forceinline__ void fold_simple(Pair (&r)[K], const uint32_t *__restrict__ a,
                                             uint32_t n_in, uint32_t n_pos, const Task &t,
                                             int k0) {
     uint32_t i = 0;
     while (i < n_in) {
-        const uint32_t ai = uniform(a[i]);
+        const uint32_t ai = uniform(sload(a, i));
+        if (!folded<kFused>(ai, t)) { i += 1; continue; }
         if (kFused && (ai & kInlineArg)) {
             Pair v[K];
             eval_fused<K>(ai, t, v);
@@ -251,8 +413,10 @@ __device__ __forceinline__ void fold_simple(Pair (&r)[K], const uint32_t *__rest
         }
         // Two rows at a time, except inside a fused gate, whose loads run
         // while the consumer's accumulator is live (register pressure).
-        const uint32_t aj = i + 1 < n_in ? uniform(a[i + 1]) : kInlineArg;
-        if (kFused && i + 1 < n_in && !(aj & kInlineArg)) {
+        uint32_t j = i + 1;
+        while (kFused && j < n_in && !folded<kFused>(uniform(sload(a, j)), t)) ++j;
+        const uint32_t aj = j < n_in ? uniform(sload(a, j)) : kInlineArg;
+        if (kFused && j < n_in && !(aj & kInlineArg)) {
             const __amdgpu_buffer_rsrc_t p0 = chunk_of(t, ai), p1 = chunk_of(t, aj);
             Pair v0[K], v1[K];
 #pragma unroll
@@ -260,14 +424,14 @@ __device__ __forceinline__ void fold_simple(Pair (&r)[K], const uint32_t *__rest
                 v0[k] = load_k(p0, t, k0 + k);
                 v1[k] = load_k(p1, t, k0 + k);
             }
-            const bool n0 = i >= n_pos, n1 = i + 1 >= n_pos;
+            const bool n0 = i >= n_pos, n1 = j >= n_pos;
             if (!n1)
                 fold2<Op, false, false>(r, v0, v1);
             else if (n0)
                 fold2<Op, true, true>(r, v0, v1);
             else
                 fold2<Op, false, true>(r, v0, v1);
-            i += 2;
+            i = j + 1;
             continue;
         }
         const __amdgpu_buffer_rsrc_t p0 = chunk_of(t, ai);
@@ -279,15 +443,17 @@ __device__ __forceinline__ void fold_simple(Pair (&r)[K], const uint32_t *__rest
     }
 }
 
-// Up to 15 inputs, pair k: four carry-save planes hold the count exactly; k8
-// above 15 can never be met (the comparison sees a k bit above the planes as
-// unmet).
+// At most 15 inputs folded, pair k: four carry-save planes hold the count
+// exactly; a threshold above 15 can never be met (the comparison sees a k
+// bit above the planes as unmet). Inputs with constant chunks are skipped
+// (k8 already counts the constant true ones).
 template <bool kFused>
 __device__ __forceinline__ Pair atleast_pair4(const uint32_t *__restrict__ a, uint32_t n_in,
                                               uint32_t n_pos, uint32_t k8, const Task &t, int k) {
     uint64_t cx[4] = {0, 0, 0, 0}, cy[4] = {0, 0, 0, 0};
     for (uint32_t i = 0; i < n_in; ++i) {
-        const uint32_t ai = uniform(a[i]);
+        const uint32_t ai = uniform(sload(a, i));
+        if (!folded<kFused>(ai, t)) continue;
         Pair v;
         if (kFused && (ai & kInlineArg)) {
             Pair f[1];
@@ -312,16 +478,24 @@ __device__ __forceinline__ Pair atleast_pair4(const uint32_t *__restrict__ a, ui
 
 // Any fan-in (row inputs only), word h of pair k: eight planes hold the
 // reference's 8-bit count, which wraps at 256 as its uint8 counter does.
+// Below 256 inputs constant chunks are skipped (k8 counts the true ones);
+// from 256 on every input is counted, a constant chunk as its constant.
 __device__ __forceinline__ uint64_t atleast_word8(const uint32_t *__restrict__ a, uint32_t n_in,
                                                   uint32_t n_pos, uint32_t k8, const Task &t, int k,
                                                   int h) {
     uint64_t c[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    const bool wide = n_in >= 256u;
     for (uint32_t i = 0; i < n_in; ++i) {
-        uint64_t v = __builtin_bit_cast(
-            uint64_t, __builtin_amdgcn_raw_buffer_load_b64(
-                          chunk_of(t, uniform(a[i])),
-                          int32_t(t.lane_off + 4096u * uint32_t(k) + 8u * uint32_t(h)), 0,
-                          kNonTemporal));
+        const uint32_t ai = uniform(sload(a, i));
+        const uint32_t st = row_summary(t, ai);
+        if (st != kChunkMixed && !wide) continue;
+        uint64_t v = st == kChunkZero ? 0ull : ~0ull;
+        if (st == kChunkMixed)
+            v = __builtin_bit_cast(
+                uint64_t, __builtin_amdgcn_raw_buffer_load_b64(
+                              chunk_of(t, ai),
+                              int32_t(t.lane_off + 4096u * uint32_t(k) + 8u * uint32_t(h)), 0,
+                              kNonTemporal));
         if (i >= n_pos) v = ~v;
 #pragma unroll
         for (int b = 0; b < 8; ++b) {
@@ -333,30 +507,29 @@ __device__ __forceinline__ uint64_t atleast_word8(const uint32_t *__restrict__ a
     return atleast_compare<uint64_t, 8>(c, k8);
 }
 
-// The gate's words for pairs k = 0..K-1 (no store, no tally of its own).
-// Fused gates appear only under AND/OR/XOR/NAND/NOR and ATLEAST with at most
-// 15 inputs, never under NOT/NULL or a wide ATLEAST.
+// The gate's words for pairs k = 0..K-1 (no store, no tally of its own),
+// for a gate that classification left to evaluate. Fused gates appear only
+// under AND/OR/XOR/NAND/NOR and ATLEAST with at most 15 inputs, never under
+// NOT/NULL or a wide ATLEAST.
 template <int K, bool kFused>
 __device__ __forceinline__ void eval_gate(const GateDesc &d, const uint32_t *__restrict__ a,
-                                          const Task &t, Pair (&r)[K]) {
+                                          const Task &t, const Cls &cls, Pair (&r)[K]) {
     const uint32_t n_in = d.counts & 0xFFFFu;
     const uint32_t n_pos = d.counts >> 16;
     const uint32_t type = d.type_k_flags & 0xFFu;
-    const uint32_t k8 = (d.type_k_flags >> 8) & 0xFFu;
     if (type == 2u) {  // ATLEAST
 #pragma unroll
         for (int k = 0; k < K; ++k) {
-            if (n_in <= 15u)
-                r[k] = atleast_pair4<kFused>(a, n_in, n_pos, k8, t, k);
+            if (n_in <= 15u || (n_in < 256u && cls.mixed <= 15u))
+                r[k] = atleast_pair4<kFused>(a, n_in, n_pos, cls.k8, t, k);
             else
-                r[k] = {atleast_word8(a, n_in, n_pos, k8, t, k, 0),
-                        atleast_word8(a, n_in, n_pos, k8, t, k, 1)};
+                r[k] = {atleast_word8(a, n_in, n_pos, cls.k8, t, k, 0),
+                        atleast_word8(a, n_in, n_pos, cls.k8, t, k, 1)};
         }
     } else if (type == 4u || type == 7u) {  // NOT, NULL: the last input
-#pragma unroll
-        for (int k = 0; k < K; ++k) r[k] = {0, 0};
+        fill<K>(r, 0ull);
         if (n_in) {
-            const __amdgpu_buffer_rsrc_t rp = chunk_of(t, uniform(a[n_in - 1]));
+            const __amdgpu_buffer_rsrc_t rp = chunk_of(t, uniform(sload(a, n_in - 1)));
 #pragma unroll
             for (int k = 0; k < K; ++k) {
                 r[k] = load_k(rp, t, k);
@@ -365,8 +538,7 @@ __device__ __forceinline__ void eval_gate(const GateDesc &d, const uint32_t *__r
         }
     } else {
         const bool and_like = (type == 0u || type == 5u);
-#pragma unroll
-        for (int k = 0; k < K; ++k) r[k] = and_like ? Pair{~0ull, ~0ull} : Pair{0, 0};
+        fill<K>(r, and_like ? ~0ull : (cls.flip ? ~0ull : 0ull));
         if (and_like)
             fold_simple<K, 0, kFused>(r, a, n_in, n_pos, t);
         else if (type == 3u)
@@ -381,28 +553,69 @@ __device__ __forceinline__ void eval_gate(const GateDesc &d, const uint32_t *__r
 }
 
 template <int K>
+__device__ __forceinline__ void store_k(const Task &t, uint32_t row, const Pair (&r)[K]) {
+    const __amdgpu_buffer_rsrc_t o = chunk_of(t, row);
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        u64x2 v;
+        v.x = r[k].x;
+        v.y = r[k].y;
+        __builtin_amdgcn_raw_buffer_store_b128(
+            __builtin_bit_cast(__attribute__((ext_vector_type(4))) unsigned, v), o,
+            int32_t(t.lane_off + 4096u * uint32_t(k)), 0, kNonTemporal);
+    }
+}
+
+template <int K>
 __device__ __forceinline__ void gate_task_k(const GateDesc &d, const uint32_t *__restrict__ a,
                                             const Task &t) {
-    Pair r[K];
-    eval_gate<K, true>(d, a, t, r);
-    if (!(d.type_k_flags & kGateNoStore)) {
-        const __amdgpu_buffer_rsrc_t o = chunk_of(t, uniform(d.out_row));
-#pragma unroll
-        for (int k = 0; k < K; ++k) {
-            u64x2 v;
-            v.x = r[k].x;
-            v.y = r[k].y;
-            __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(__attribute__((ext_vector_type(4))) unsigned, v), o,
-                                                   int32_t(t.lane_off + 4096u * uint32_t(k)), 0,
-                                                   kNonTemporal);
+    const uint32_t n_in = d.counts & 0xFFFFu;
+    const uint32_t out = uniform(d.out_row);
+    const bool store = !(d.type_k_flags & kGateNoStore);
+    const bool summarise = t.L->state && store;
+    const Cls cls = classify<true>(d.type_k_flags & 0xFFu, (d.type_k_flags >> 8) & 0xFFu, n_in,
+                                   d.counts >> 16, a, t);
+    tally_constant_fused(a, n_in, t);
+    if (cls.konst >= 0) {
+        // Decided by summaries alone: no input is loaded. Fused inputs whose
+        // chunks are not constant are still evaluated, for their tallies.
+        for (uint32_t i = 0; i < n_in; ++i) {
+            const uint32_t ai = uniform(sload(a, i));
+            if ((ai & kInlineArg) && folded<true>(ai, t)) {
+                Pair v[K];
+                eval_fused<K>(ai, t, v);
+            }
         }
+        if (d.type_k_flags & kGateTallied) tally_constant(t, out, cls.konst);
+        if (d.type_k_flags & kGateForceStore) {
+            Pair r[K];
+            fill<K>(r, cls.konst ? ~0ull : 0ull);
+            store_k<K>(t, out, r);
+        }
+        if (summarise) put_summary(*t.L, out, t.chunk, uint32_t(cls.konst));
+        return;
     }
+    Pair r[K];
+    eval_gate<K, true>(d, a, t, cls, r);
     if (d.type_k_flags & kGateTallied) {
         uint32_t pc = 0;
 #pragma unroll
         for (int k = 0; k < K; ++k) pc += pair_popcount(t, r[k], k);
-        tally_wave(t.ones, d.out_row, pc);
+        tally_wave(t.ones, out, pc);
     }
+    if (!store) return;
+    uint32_t st = kChunkMixed;
+    if (summarise) {
+        uint64_t any = 0, all = ~0ull;
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            if (t.full || lane_word(t, k, 0) < t.n_bp) { any |= r[k].x; all &= r[k].x; }
+            if (t.full || lane_word(t, k, 1) < t.n_bp) { any |= r[k].y; all &= r[k].y; }
+        }
+        st = chunk_summary(any != 0, all != ~0ull);
+        put_summary(*t.L, out, t.chunk, st);
+    }
+    if (st == kChunkMixed || (d.type_k_flags & kGateForceStore)) store_k<K>(t, out, r);
 }
 
 // Gate task: one gate over words [base, base + chunk_words).
@@ -415,8 +628,10 @@ __device__ __forceinline__ void gate_task(const PassLaunch &L, uint64_t *__restr
     const GateDesc d = L.gates[gi];
     const uint32_t *a = L.arg_rows + d.arg_begin;
     const uint64_t base = uint64_t(chunk) * geo.chunk_words;
+    const uint64_t live = geo.n_bp - base < geo.chunk_words ? geo.n_bp - base : geo.chunk_words;
     const Task t{&L, words, ones, geo.stride, base, geo.n_bp, 16u * threadIdx.x,
-                 8u * geo.chunk_words, base + geo.chunk_words <= geo.n_bp};
+                 8u * geo.chunk_words, base + geo.chunk_words <= geo.n_bp, chunk,
+                 uint32_t(live)};
     switch (geo.chunk_words) {
     case 2048: gate_task_k<4>(d, a, t); break;
     case 1024: gate_task_k<2>(d, a, t); break;
