@@ -7,14 +7,17 @@ every gate tallied, 2^18 uint64 words = 2^24 trials per pass. One step is one
 pass (Philox sampling of every basic event, every gate layer, tallies);
 2^32 trials are 256 steps.
 
-Prints one JSON line: gate-evals/s (gates x trials / s), the HBM roofline of
-the dominant kernel measured with HIP events on the engine's stream, and the
+Prints one JSON line: gate-evals/s (gates x trials / s), the roofline of
+the dominant kernel (its launch time measured with HIP events on the
+engine's stream; its VALU instructions and HBM bytes per launch from the
+committed rocprofv3 counter summaries of the same kernel source), and the
 CPU oracle timed on the host on a bounded sample of the same tree (with a
 bit-exact tally comparison against the GPU on that sample).
 """
 from __future__ import annotations
 
 import argparse
+import hashlib
 import json
 import os
 import sys
@@ -23,6 +26,24 @@ import time
 import numpy as np
 
 HBM_PEAK_GBS = 8000.0
+# VALU issue peak: one wave64 instruction per CU per clock (four SIMDs of
+# 16 lanes, four cycles each), 256 CUs at the 2.4 GHz peak engine clock:
+# 39.3 T int32 lane operations per second.
+CUS, PEAK_CLOCK_GHZ = 256, 2.4
+VALU_PEAK_TOPS = CUS * 64 * PEAK_CLOCK_GHZ / 1e3
+ROOT = os.path.dirname(os.path.abspath(__file__))
+KERNEL_SOURCES = ("mcscram_amd/csrc/kernels.hip", "mcscram_amd/csrc/kernels.h",
+                  "mcscram_amd/csrc/device_ops.h")
+
+
+def kernel_source_sha16() -> str:
+    """Hash of pass_layer's source files: counter summaries recorded with the
+    same hash describe the kernel that runs."""
+    h = hashlib.sha256()
+    for f in KERNEL_SOURCES:
+        with open(os.path.join(ROOT, f), "rb") as fh:
+            h.update(fh.read())
+    return h.hexdigest()[:16]
 
 
 def algorithmic_bytes(pd, n_bp: int) -> dict:
@@ -63,12 +84,12 @@ def host_cpu() -> dict:
     return {"cpu_model": model, "logical_cpus": os.cpu_count()}
 
 
-def measured_traffic(tree: str, n_bp: int):
-    """HBM bytes per launch of pass_layer from the committed rocprofv3
-    FETCH_SIZE / WRITE_SIZE summary of this workload (scaled by the copy
-    calibration recorded there), or None when no summary matches."""
-    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
-                        f"pass_layer_{tree}_tree_hbm_counters.json")
+def counter_summary(kind: str, tree: str, n_bp: int):
+    """The committed rocprofv3 counter summary of pass_layer on this workload
+    (kind "hbm": FETCH_SIZE / WRITE_SIZE scaled by the copy calibration;
+    "valu": SQ_INSTS_VALU and the clock), or None when none matches the
+    workload or the kernel source."""
+    path = os.path.join(ROOT, "profiles", f"pass_layer_{tree}_tree_{kind}_counters.json")
     try:
         with open(path) as f:
             s = json.load(f)
@@ -76,7 +97,9 @@ def measured_traffic(tree: str, n_bp: int):
         return None
     if s.get("words_per_pass") != n_bp or s.get("tree") != tree:
         return None
-    return s["traffic_bytes_per_launch"]
+    if s.get("kernel_source_sha16") != kernel_source_sha16():
+        return None
+    return s
 
 
 def copy_calibration(n_bytes: int) -> dict:
@@ -166,68 +189,90 @@ def bench_fused(args, mc) -> int:
 
 def bench_ua(args) -> int:
     """Surrogate-screened uncertainty analysis against plain batched GPU Monte
-    Carlo over the same held-out parameter samples.
+    Carlo.
 
     ua: the 2,000-event / 3,000-gate tree (lognormal basic events, median
-    log-uniform in [1e-5, 1e-2], error factor uniform in [3, 10]); training
-    pairs and outer samples at 2^22 trials, 16,384 outer samples.
-    ua-large: the large tree's gate 64134 (956 events, 210 gates; the tree's
-    own top event is certain), lognormal around the tree's probabilities with
-    error factor uniform in [3, 10]; training pairs at 2^22 trials, 100,000
-    outer samples at 2^24 trials, plain Monte Carlo over all of them."""
+    log-uniform in [1e-5, 1e-2], error factor uniform in [3, 10]); 65,536
+    training pairs and 16,384 outer samples at 2^22 trials; plain Monte Carlo
+    over every outer sample is the yardstick.
+    ua-scale: the large tree's family (25,000 events, 40,002 gates; top
+    AND(A, B) over the last layer, p about 0.02) with lognormal uncertainty
+    (error factor uniform in [3, 10]) on every event, all 25,000 the network's
+    inputs; training pairs at 2^20 trials; 100,000 outer samples, routed ones
+    (disagreement over 0.05 in logit, plus a random 5 % audit) at 2^24 trials.
+    Plain Monte Carlo over all 100,000 would take hours: its rate is the
+    routed runs' (same engine, same shape) and the surrogate's error is
+    measured paired, prediction against Monte Carlo, on the random audit."""
     from mcscram_amd import screening as scr
     from mcscram_amd import ua
-    large = args.workload == "ua-large"
-    if large:
-        pd, model = ua.large_tree_uncertainty()
+    scale = args.workload == "ua-scale"
+    if scale:
+        pd, model = ua.at_scale_uncertainty()
         outer = args.ua_outer or 100_000
         words = args.ua_words or 1 << 18
-        name = (f"surrogate-screened UA, large tree's gate {ua.LARGE_TREE_UA_TOP} "
-                f"({pd.n_vars} events / {pd.n_gates} gates)")
+        train = args.ua_train or 32768
+        train_words = args.ua_train_words or 1 << 14
+        name = (f"surrogate-screened UA at scale: {pd.n_vars} events / {pd.n_gates} gates "
+                f"(large tree's family, top AND of the last layer's rare and likely gates)")
     else:
         pd, model = ua.uncertainty_tree()
         outer = args.ua_outer or 16384
         words = args.ua_words or 1 << 16
+        train = args.ua_train or 65536
+        train_words = args.ua_train_words or 1 << 16
         name = "surrogate-screened UA, 2,000 events / 3,000 gates"
-    train_words = args.ua_train_words or 1 << 16
-    epochs = args.ua_epochs or (60 if large else 30)
+    epochs = args.ua_epochs or (60 if scale else 30)
     say = lambda m: print(f"[ua] {m}", file=sys.stderr, flush=True)  # noqa: E731
-    say(f"training pairs: {args.ua_train} x {train_words * 64} trials")
-    ens, info = scr.train_surrogate(pd, model, args.ua_train, train_words, seed=3,
-                                    epochs=epochs,
+    say(f"training pairs: {train} x {train_words * 64} trials")
+    ens, info = scr.train_surrogate(pd, model, train, train_words, seed=3, epochs=epochs,
+                                    chunk=4096 if scale else 16384,
                                     progress=lambda d, n: say(f"training pairs {d}/{n}"))
     say(f"trained: {info}")
-    res = scr.screened_uncertainty(pd, model, ens, outer, words, seed=1234)
-    say(f"screened: {res.seconds}, routed {res.routed.mean():.3f}")
-    plain, t_plain = scr.plain_uncertainty(pd, model, outer, words, seed=1234, chunk=8192,
-                                           progress=lambda d, n: say(f"plain Monte Carlo {d}/{n}"))
-    say(f"plain Monte Carlo: {t_plain:.1f} s")
-    cmp = scr.compare(res, plain)
+    res = scr.screened_uncertainty(pd, model, ens, outer, words, seed=1234,
+                                   mc_chunk=256 if scale else 1 << 30,
+                                   progress=lambda d, n: say(f"routed Monte Carlo {d}/{n}"))
+    say(f"screened: {res.seconds}, routed {res.routed.mean():.4f}")
     t_scr = res.seconds["total"]
     out = {
         "metric": "UA parameter-samples/s at <=2% rel err vs plain MC",
         "value": outer / t_scr, "unit": "samples/s", "n_gpus": 1, "steps": 1,
         "warmup": 0, "ms_per_step": t_scr * 1e3, "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "bf16 network, u64 Monte Carlo", "data": "synthetic",
-        "config": {"workload": name, "training_pairs": args.ua_train,
+        "config": {"workload": name, "training_pairs": train,
                    "trials_per_training_pair": train_words * 64, "outer_samples": outer,
                    "trials_per_mc_run": words * 64,
                    "network": f"{pd.n_vars}-512-512-512-1 GELU x2", "epochs": epochs,
-                   "disagreement": 0.05, "audit": 0.05},
-        "plain_mc_samples_per_s": outer / t_plain,
-        "speedup_vs_plain_mc": t_plain / t_scr,
+                   "disagreement_logit": 0.05, "audit": 0.05},
         # The one-off cost of the training pairs and the fit, charged to this
         # one set of outer samples (it is shared by every later analysis of
         # the same tree and uncertainty model).
         "training_seconds": info["mc_seconds"] + info["fit_seconds"],
-        "speedup_including_training": t_plain / (t_scr + info["mc_seconds"] + info["fit_seconds"]),
         # The random audit sends this fraction to Monte Carlo whatever the
         # ensemble says, so the speedup cannot exceed 1 / audit.
         "speedup_cap_from_audit": 1.0 / 0.05,
-        "comparison": cmp, "audit": res.audit_rel_error,
+        "routed_fraction": float(res.routed.mean()),
+        "disagreed_fraction": float(res.disagreed.mean()),
+        "audit_paired": res.audit_rel_error,
+        "screened_statistics": scr._quantiles(res.estimates),
         "training": info,
-        "within_2pct": bool(cmp["mean"]["rel_error"] <= 0.02 and cmp["p95"]["rel_error"] <= 0.02),
     }
+    if scale:
+        per = res.seconds["mc_per_sample"]
+        t_plain = per * outer
+        out["plain_mc_samples_per_s"] = 1.0 / per
+        out["plain_mc"] = ("rate of the routed runs (%d samples, same engine and shape); all "
+                           "%d samples would take %.0f s" % (int(res.routed.sum()), outer, t_plain))
+        cmp = res.audit_rel_error
+    else:
+        plain, t_plain = scr.plain_uncertainty(pd, model, outer, words, seed=1234, chunk=8192,
+                                               progress=lambda d, n: say(f"plain Monte Carlo {d}/{n}"))
+        say(f"plain Monte Carlo: {t_plain:.1f} s")
+        cmp = scr.compare(res, plain)
+        out["plain_mc_samples_per_s"] = outer / t_plain
+        out["comparison"] = cmp
+    out["speedup_vs_plain_mc"] = t_plain / t_scr
+    out["speedup_including_training"] = t_plain / (t_scr + out["training_seconds"])
+    out["within_2pct"] = bool(cmp["mean"]["rel_error"] <= 0.02 and cmp["p95"]["rel_error"] <= 0.02)
     print(json.dumps(out), flush=True)
     return 0
 
@@ -252,24 +297,24 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--words", type=int, default=1 << 18, help="uint64 words per pass")
     ap.add_argument("--tree", choices=["large", "small"], default="large")
-    ap.add_argument("--workload", choices=["layered", "batched", "small-fused", "ua", "ua-large"],
+    ap.add_argument("--workload", choices=["layered", "batched", "small-fused", "ua", "ua-scale"],
                     default="layered",
                     help="layered: the large tree through the layered engine (default); "
                          "batched: 256 uneven trees, 2^26 trials each, one persistent launch "
                          "per pass; small-fused: the small tree, 2^20 trials, fused kernel; "
                          "ua: surrogate-screened uncertainty analysis vs plain GPU Monte Carlo; "
-                         "ua-large: the same on the large tree's gate 64134, 100,000 outer "
+                         "ua-scale: the same on the large tree's family, 100,000 outer "
                          "samples at 2^24 trials")
-    ap.add_argument("--ua-train", type=int, default=65536, help="training pairs (ua)")
+    ap.add_argument("--ua-train", type=int, default=0,
+                    help="training pairs (ua: 65,536; ua-scale: 32,768)")
     ap.add_argument("--ua-outer", type=int, default=0,
-                    help="outer parameter samples (ua: 16,384; ua-large: 100,000)")
+                    help="outer parameter samples (ua: 16,384; ua-scale: 100,000)")
     ap.add_argument("--ua-words", type=int, default=0,
-                    help="words per outer Monte Carlo run (ua: 2^16; ua-large: 2^18)")
+                    help="words per outer Monte Carlo run (ua: 2^16; ua-scale: 2^18)")
     ap.add_argument("--ua-train-words", type=int, default=0,
-                    help="words per training Monte Carlo run (2^16)")
+                    help="words per training Monte Carlo run (ua: 2^16; ua-scale: 2^14)")
     ap.add_argument("--ua-epochs", type=int, default=0,
-                    help="training epochs (ua: 30; ua-large: 60, which cuts the members' "
-                         "disagreement on its 956 inputs from 0.7 % to 0.2 % of samples)")
+                    help="training epochs (ua: 30; ua-scale: 60)")
     ap.add_argument("--trees", type=int, default=256)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
@@ -287,7 +332,7 @@ def main() -> int:
         else:
             args.steps = 3 if args.workload == "batched" else 32
     import mcscram_amd as mc
-    if args.workload in ("ua", "ua-large"):
+    if args.workload in ("ua", "ua-scale"):
         return bench_ua(args)
     if args.workload != "layered":
         return bench_fused(args, mc)
@@ -324,6 +369,39 @@ def main() -> int:
     bytes_per_launch = ab["pass"] / launches_per_pass
     achieved = bytes_per_launch / launch_s / 1e9
     pass_gbs = ab["pass"] / (wall / args.steps) / 1e9
+    # Counters of the same kernel source on the same workload: HBM bytes
+    # and VALU instructions per launch, over this run's launch time.
+    hbm_c = counter_summary("hbm", args.tree, n_bp)
+    valu_c = counter_summary("valu", args.tree, n_bp)
+    traffic = hbm_c["traffic_bytes_per_launch"] if hbm_c else None
+    roofline = {"kernel": "pass_layer", "bound": "hbm", "achieved": achieved,
+                "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                "traffic": traffic}
+    hbm = {"algorithmic_bytes_per_launch": bytes_per_launch, "algorithmic_achieved": achieved,
+           "algorithmic_frac": achieved / HBM_PEAK_GBS, "peak": HBM_PEAK_GBS, "unit": "GB/s"}
+    if traffic:
+        hbm["counter_achieved"] = traffic / launch_s / 1e9
+        hbm["counter_frac"] = hbm["counter_achieved"] / HBM_PEAK_GBS
+    if valu_c:
+        instr = valu_c["valu_instructions_per_launch"]
+        tops = instr * 64 / launch_s / 1e12
+        clock = valu_c["clock_ghz"]
+        valu = {"instructions_per_launch": instr, "achieved": tops, "peak": VALU_PEAK_TOPS,
+                "unit": "TOP/s", "frac": tops / VALU_PEAK_TOPS,
+                # SQ_INSTS_VALU / (256 CUs x the clock the counters measured x time)
+                "achieved_valu": instr / (CUS * clock * 1e9 * launch_s), "clock_ghz": clock}
+        # The binding roofline is the one the counters put closest to its peak.
+        if valu["frac"] >= hbm.get("counter_frac", hbm["algorithmic_frac"]):
+            roofline = {"kernel": "pass_layer", "bound": "valu", "achieved": tops,
+                        "peak": VALU_PEAK_TOPS, "unit": "TOP/s", "frac": tops / VALU_PEAK_TOPS,
+                        "traffic": traffic}
+        roofline["valu"] = valu
+        roofline["achieved_valu"] = valu["achieved_valu"]
+    roofline["hbm"] = hbm
+    roofline.update({"avg_launch_ms": launch_s * 1e3, "launches_per_pass": launches_per_pass,
+                     "counters": "profiles/pass_layer_%s_tree_{hbm,valu}_counters.json" % args.tree
+                     if (hbm_c and valu_c) else "no summary for this kernel source",
+                     "kernel_source_sha16": kernel_source_sha16()})
 
     out = {
         "metric": "bit-level gate-evals/s",
@@ -344,13 +422,10 @@ def main() -> int:
                    "trials_per_pass": n_bp * 64, "trials_timed": trials},
         "node_evals_per_s": (pd.n_vars + pd.n_gates) * trials / wall,
         # The metric's second part: the dominant kernel's share of HBM peak
-        # (the roofline's achieved / peak, in percent).
+        # on the materialised dataflow's bytes, and on the bytes it moves.
         "hbm_peak_pct": 100.0 * achieved / HBM_PEAK_GBS,
-        "roofline": {"kernel": "pass_layer", "bound": "hbm", "achieved": achieved,
-                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": measured_traffic(args.tree, n_bp),
-                     "algorithmic_bytes_per_launch": bytes_per_launch,
-                     "avg_launch_ms": launch_s * 1e3, "launches_per_pass": launches_per_pass},
+        "hbm_peak_pct_measured_traffic": 100.0 * hbm["counter_frac"] if traffic else None,
+        "roofline": roofline,
         "pass_roofline": {"bound": "hbm", "algorithmic_bytes_per_pass": ab["pass"],
                           "achieved": pass_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                           "frac": pass_gbs / HBM_PEAK_GBS},

@@ -81,6 +81,8 @@ struct LayeredGroup {
     mcs::GateDesc *d_gates = nullptr;
     uint32_t *d_args[2] = {nullptr, nullptr};
     unsigned long long *d_ones = nullptr;  // leading pad of 2 * max_ev, then the trees
+    uint8_t *d_sum = nullptr;              // chunk summaries of the arena's rows
+    uint32_t sum_stride = 0;
     uint64_t seq = 0;                      // tree-passes run: event area = seq & 1
     int64_t primed_pos = -1;               // tree position sampled into area seq & 1
     uint32_t primed_iter = 0;
@@ -156,6 +158,8 @@ mcs_status enqueue_layered(mcs_batch *b, uint32_t it) {
             base.key0 = b->cfg.key0;
             base.key1 = b->cfg.key1;
             base.iteration_ptr = b->d_state + 1;
+            base.state = g.d_sum;
+            base.state_stride = g.sum_stride;
             if (!(g.primed_pos == int64_t(pos) && g.primed_iter == it)) {
                 mcs::PassLaunch pl = base;
                 pl.events = g.d_events + tr.ev_begin;
@@ -604,8 +608,12 @@ mcs_status build_batch(const mcs_pdag *pdags, int32_t n_trees, const mcs_sample_
         MCS_HIP(hipMalloc(&g.d_ones, ones_total * 8));
         keep(g.d_ones);
         MCS_HIP(hipMemsetAsync(g.d_ones, 0, ones_total * 8, b->stream));
+        g.sum_stride = mcs::state_stride_for(geo.n_chunks);
+        MCS_HIP(hipMalloc(&g.d_sum, rows * g.sum_stride));
+        keep(g.d_sum);
+        MCS_HIP(hipMemsetAsync(g.d_sum, mcs::kChunkMixed, rows * g.sum_stride, b->stream));
         b->device_bytes += word_bytes + gd.size() * sizeof(mcs::GateDesc) + (a0.size() + a1.size()) * 4 +
-                           evs.size() * sizeof(mcs::EventDesc) + ones_total * 8;
+                           evs.size() * sizeof(mcs::EventDesc) + ones_total * 8 + rows * g.sum_stride;
     }
     MCS_HIP(hipStreamSynchronize(b->stream));
 

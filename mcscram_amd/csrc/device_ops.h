@@ -128,9 +128,12 @@ __device__ __forceinline__ uint64_t bernoulli_word_t(uint32_t c0, uint32_t c1, u
     return uint64_t(hi_bits) << 32 | lo_bits;
 }
 
-// Thresholds at or above this (p >= 2^-10) make hits frequent enough that
-// the branch-free packing is cheaper for a wave holding such an event.
-constexpr uint32_t kDenseThreshold = 1u << 22;
+// The branch-free packing costs about 43.5 VALU instructions per Philox
+// block, the branching one 35 plus 11 times the chance P that some lane of
+// the wave hits in a generation, P = 1 - (1 - p)^256. They break even near
+// P = 0.78, p = 5.9e-3: thresholds at or above 1.5 x 2^24 take the dense
+// packing.
+constexpr uint32_t kDenseThreshold = (1u << 24) + (1u << 23);
 
 // The Bernoulli word with the packing chosen per wave.
 __device__ __forceinline__ uint64_t bernoulli_word(uint32_t c0, uint32_t c1, uint32_t c2,

@@ -98,6 +98,7 @@ __device__ __forceinline__ void sampler_task(const PassLaunch &L, uint64_t *__re
     const uint32_t c3base = (iteration + 1u) << 6;
     uint64_t *row = words + uint64_t(L.samp_row0 + e) * geo.stride;
     const uint64_t base = uint64_t(chunk) * geo.chunk_words;
+    const bool full = base + geo.chunk_words <= geo.n_bp;  // no padding word
     uint64_t any = 0, all = ~0ull;  // over the lane's live words
     for (uint32_t off = threadIdx.x * 2u; off < geo.chunk_words; off += 2u * kLanes) {
         const uint64_t s = base + off;
@@ -110,8 +111,13 @@ __device__ __forceinline__ void sampler_task(const PassLaunch &L, uint64_t *__re
         }
         store_pair(row + s, w0, w1);
         if (L.state) {
-            if (s < geo.n_bp) { any |= w0; all &= w0; }
-            if (s + 1 < geo.n_bp) { any |= w1; all &= w1; }
+            if (full) {
+                any |= w0 | w1;
+                all &= w0 & w1;
+            } else {
+                if (s < geo.n_bp) { any |= w0; all &= w0; }
+                if (s + 1 < geo.n_bp) { any |= w1; all &= w1; }
+            }
         }
     }
     if (L.state) {

@@ -43,6 +43,7 @@ struct UaPipe {
     hipStream_t stream = nullptr;
     hipEvent_t done = nullptr;           // recorded after a run's launches
     uint64_t *d_words = nullptr;         // the pipeline's words arena
+    uint8_t *d_sum = nullptr;            // chunk summaries of the arena's rows
     uint32_t k_begin = 0, k_end = 0;     // its samples
     uint64_t seq = 0;                    // sample-passes run: event area = seq & 1
     int64_t primed = -1;                 // sample sampled into area seq & 1 ...
@@ -86,6 +87,7 @@ struct mcs_ua {
             if (p) (void)hipFree(p);
         for (UaPipe &q : pipes) {
             if (q.d_words) (void)hipFree(q.d_words);
+            if (q.d_sum) (void)hipFree(q.d_sum);
             if (q.done) (void)hipEventDestroy(q.done);
             if (q.stream) (void)hipStreamDestroy(q.stream);
         }
@@ -110,6 +112,8 @@ mcs_status enqueue_run(mcs_ua *u, uint32_t n_passes) {
                 const uint32_t area = uint32_t(q.seq & 1u);
                 mcs::PassLaunch base{};
                 base.key1 = u->cfg.key1;
+                base.state = q.d_sum;
+                base.state_stride = mcs::state_stride_for(u->geo.n_chunks);
                 if (!(q.primed == int64_t(k) && q.primed_iter == it)) {
                     mcs::PassLaunch pl = base;
                     pl.events = u->d_events + uint64_t(k) * u->n_ev;
@@ -244,7 +248,8 @@ mcs_status mcs_ua_create(const mcs_pdag *pdag, const mcs_sample_shape *shape, in
     MCS_HIP(hipGetDevice(&u->device));
     MCS_HIP(hipStreamCreateWithFlags(&u->stream, hipStreamNonBlocking));
     const uint64_t rows = 2ull * u->n_ev + u->gate_rows;
-    const uint64_t word_bytes = rows * geo.stride * 8;
+    const uint64_t sum_bytes = rows * mcs::state_stride_for(geo.n_chunks);
+    const uint64_t word_bytes = rows * geo.stride * 8 + sum_bytes;  // words and summaries
     const uint64_t ones_n = 2ull * u->n_ev + uint64_t(n_samples) * u->slots;
     size_t free_b = 0, total_b = 0;
     MCS_HIP(hipMemGetInfo(&free_b, &total_b));
@@ -265,7 +270,9 @@ mcs_status mcs_ua_create(const mcs_pdag *pdag, const mcs_sample_shape *shape, in
         q.k_end = uint32_t(uint64_t(n_samples) * (i + 1) / n_pipes);
         MCS_HIP(hipStreamCreateWithFlags(&q.stream, hipStreamNonBlocking));
         MCS_HIP(hipEventCreateWithFlags(&q.done, hipEventDisableTiming));
-        MCS_HIP(hipMalloc(&q.d_words, word_bytes));
+        MCS_HIP(hipMalloc(&q.d_words, word_bytes - sum_bytes));
+        MCS_HIP(hipMalloc(&q.d_sum, sum_bytes));
+        MCS_HIP(hipMemsetAsync(q.d_sum, mcs::kChunkMixed, sum_bytes, u->stream));
     }
     MCS_HIP(hipMalloc(&u->d_events, ev.size() * sizeof(mcs::EventDesc)));
     MCS_HIP(hipMalloc(&u->d_gates, std::max<size_t>(u->lp.gates.size(), 1) * sizeof(mcs::GateDesc)));

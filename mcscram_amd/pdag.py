@@ -18,6 +18,7 @@ same tree on every platform (no library distribution is involved).
 """
 from __future__ import annotations
 
+import bisect
 import math
 from dataclasses import dataclass, field
 
@@ -258,12 +259,18 @@ def random_layered_tree(seed: int, n_be: int, n_gates: int, n_layers: int,
 
     if top == "last_layer":
         last = max(layer_of)
+        # Gates are generated layer by layer, so the gates of layers above l
+        # are the index range from first_above[l] on; AND gates likewise.
+        first_above = {l: bisect.bisect_right(layer_of, l) for l in set(layer_of)}
+        ands_below_last = [j for j in range(len(gates))
+                           if gates[j][1] == AND and layer_of[j] < last]
+        ands_all = [j for j in range(len(gates)) if gates[j][1] == AND]
         for gi, g in enumerate(gates):
             if layer_of[gi] == last or g[0] in used:
                 continue
-            higher = [j for j in range(len(gates)) if layer_of[j] > layer_of[gi]]
-            ands = [j for j in higher if gates[j][1] == AND and layer_of[j] < last] or \
-                [j for j in higher if gates[j][1] == AND] or higher
+            lo = first_above[layer_of[gi]]
+            ands = ands_below_last[bisect.bisect_left(ands_below_last, lo):] or \
+                ands_all[bisect.bisect_left(ands_all, lo):] or range(lo, len(gates))
             j = ands[rng.randint(0, len(ands) - 1)]
             gates[j][3].append(g[0])
             used.add(g[0])

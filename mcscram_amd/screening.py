@@ -11,10 +11,11 @@
    2,000-event uncertainty tree a linear fit of the logit explains 35 % of its
    variance on p against 1 % on ln p.
 3. Screening: every outer sample is predicted by both members; samples whose
-   members disagree by more than ``disagreement`` in logit space (relative
-   to the logit's magnitude), plus a random ``audit`` fraction, go to plain
-   Monte Carlo. Routed samples take the Monte Carlo value; the audit measures
-   the surrogate's error on samples it was trusted with.
+   members' logits differ by more than ``disagreement`` (an absolute bound in
+   logit space: 0.05 is a 5 % ratio of the two predicted odds, whatever the
+   probability), plus a random ``audit`` fraction, go to plain Monte Carlo.
+   Routed samples take the Monte Carlo value; the audit measures the
+   surrogate's error on samples it was trusted with.
 
 Plain Monte Carlo over every outer sample is the fallback and the yardstick of
 the result (``plain_uncertainty``). Monte Carlo estimates use
@@ -83,13 +84,14 @@ def train_surrogate(pd: Pdag, model: LognormalModel, n_train: int, words: int, s
     """Monte Carlo training pairs (in runs of ``chunk`` samples, each followed
     by ``progress(done, n_train)`` if given) and a fitted two-member ensemble.
     -> (ensemble, info dict with timings and the training losses)."""
-    p, z = model.sample(n_train, seed)
     ids = (id_base + np.arange(n_train)).astype(np.uint32)
     p_hat = np.empty(n_train)
+    z = np.empty((n_train, pd.n_vars), dtype=np.float32)
     t_mc = 0.0
-    for a in range(0, n_train, chunk):
-        b = min(n_train, a + chunk)
-        p_hat[a:b], dt = monte_carlo(pd, p[a:b], words, ids[a:b])
+    for a, p, zc in model.sample_chunks(n_train, seed, chunk):
+        b = a + p.shape[0]
+        z[a:b] = zc
+        p_hat[a:b], dt = monte_carlo(pd, p, words, ids[a:b])
         t_mc += dt
         if progress:
             progress(b, n_train)
@@ -102,40 +104,75 @@ def train_surrogate(pd: Pdag, model: LognormalModel, n_train: int, words: int, s
                  "final_losses": [float(l[-1]) for l in losses]}
 
 
+def route(lg: np.ndarray, disagreement: float) -> np.ndarray:
+    """Samples whose ensemble members' logits lg [members, n] differ by more
+    than ``disagreement`` (absolute, in logit space)."""
+    return (lg.max(axis=0) - lg.min(axis=0)) > disagreement
+
+
+def _quantiles(x):
+    return {"mean": float(np.mean(x)), "p05": float(np.quantile(x, 0.05)),
+            "p50": float(np.median(x)), "p95": float(np.quantile(x, 0.95))}
+
+
+def paired_errors(surrogate: np.ndarray, mc: np.ndarray) -> dict:
+    """Surrogate predictions against Monte Carlo on the same samples: the
+    relative error of the mean, 5th, 50th and 95th percentile, and the
+    median absolute logit error."""
+    s, m = _quantiles(surrogate), _quantiles(mc)
+    out = {k: {"surrogate": s[k], "plain": m[k], "rel_error": abs(s[k] - m[k]) / m[k]}
+           for k in s}
+    out["samples"] = int(len(mc))
+    out["median_abs_logit_error"] = float(np.median(np.abs(logit(surrogate) - logit(mc))))
+    return out
+
+
 def screened_uncertainty(pd: Pdag, model: LognormalModel, ens: Ensemble, n_outer: int,
                          words: int, seed: int, disagreement: float = 0.05, audit: float = 0.05,
-                         id_base: int = 0) -> ScreeningResult:
-    p, z = model.sample(n_outer, seed)
-    t0 = time.perf_counter()
-    lg = ens.predict(z)                       # [2, n_outer]
-    t_pred = time.perf_counter() - t0
-    mean_l = lg.mean(axis=0)
-    spread = np.abs(lg[0] - lg[1])
-    disagreed = spread > disagreement * np.maximum(np.abs(mean_l), 1.0)
+                         id_base: int = 0, chunk: int = 16384, mc_chunk: int = 1 << 30,
+                         progress=None) -> ScreeningResult:
+    """Predict every outer sample (in blocks of ``chunk`` samples), route the
+    disagreeing and audited ones to plain Monte Carlo (in runs of
+    ``mc_chunk`` samples, each followed by ``progress(done, n_routed)``) and
+    keep the Monte Carlo value for them."""
     rng = np.random.Generator(np.random.PCG64(seed ^ 0xA0D17))
     audited = rng.random(n_outer) < audit
+    disagreed = np.zeros(n_outer, dtype=bool)
+    est = np.empty(n_outer)
+    routed_p = []
+    t0 = time.perf_counter()
+    t_pred = 0.0
+    for a, p, z in model.sample_chunks(n_outer, seed, chunk):
+        b = a + p.shape[0]
+        t = time.perf_counter()
+        lg = ens.predict(z)                   # [members, rows]
+        t_pred += time.perf_counter() - t
+        disagreed[a:b] = route(lg, disagreement)
+        est[a:b] = sigmoid(lg.mean(axis=0))
+        r = disagreed[a:b] | audited[a:b]
+        routed_p.append(p[r])
     routed = disagreed | audited
-    est = sigmoid(mean_l)
     idx = np.nonzero(routed)[0]
     t_mc = 0.0
     audit_err = {}
     if idx.size:
-        p_hat, t_mc = monte_carlo(pd, p[idx], words, (id_base + idx).astype(np.uint32))
+        p_r = np.concatenate(routed_p)
+        p_hat = np.empty(idx.size)
+        for a in range(0, idx.size, mc_chunk):
+            b = min(idx.size, a + mc_chunk)
+            p_hat[a:b], dt = monte_carlo(pd, p_r[a:b], words, (id_base + idx[a:b]).astype(np.uint32))
+            t_mc += dt
+            if progress:
+                progress(b, idx.size)
         surrogate_vals = est[idx].copy()
         est[idx] = p_hat
         a = audited[idx] & ~disagreed[idx]
         if a.any():
-            s_a, m_a = surrogate_vals[a], p_hat[a]
-            audit_err = {
-                "samples": int(a.sum()),
-                "mean": float(abs(s_a.mean() - m_a.mean()) / m_a.mean()),
-                "p95": float(abs(np.quantile(s_a, 0.95) - np.quantile(m_a, 0.95)) /
-                             np.quantile(m_a, 0.95)),
-                "median_abs_logit_error": float(np.median(np.abs(logit(s_a) - logit(m_a)))),
-            }
+            audit_err = paired_errors(surrogate_vals[a], p_hat[a])
     return ScreeningResult(est, routed, disagreed, audited, audit_err,
                            {"predict": t_pred, "monte_carlo": t_mc,
-                            "total": time.perf_counter() - t0})
+                            "total": time.perf_counter() - t0,
+                            "mc_per_sample": t_mc / idx.size if idx.size else None})
 
 
 def plain_uncertainty(pd: Pdag, model: LognormalModel, n_outer: int, words: int, seed: int,

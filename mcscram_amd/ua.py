@@ -22,7 +22,7 @@ from dataclasses import dataclass
 import numpy as np
 
 from .engine import _check, _p, _PdagView, _shape, load_library, MCS_REFERENCE_KEY
-from .pdag import Pdag, SplitMix64, large_tree, random_layered_tree
+from .pdag import Pdag, SplitMix64, random_layered_tree
 
 Z95 = 1.6448536269514722  # standard normal quantile of 0.95
 
@@ -116,6 +116,16 @@ class LognormalModel:
         p = np.exp(self.mu[None, :] + self.sigma[None, :] * z)
         return np.clip(p, 0.0, 1.0), z
 
+    def sample_chunks(self, n: int, seed: int, chunk: int):
+        """The samples of ``sample(n, seed)`` in consecutive row blocks of at
+        most ``chunk`` rows (one generator, drawn in order, so the rows are
+        the same). Yields (first row, p, z)."""
+        rng = np.random.Generator(np.random.PCG64(seed))
+        for a in range(0, n, chunk):
+            z = rng.standard_normal((min(chunk, n - a), self.median.shape[0]))
+            p = np.exp(self.mu[None, :] + self.sigma[None, :] * z)
+            yield a, np.clip(p, 0.0, 1.0), z
+
 
 def uncertainty_tree(seed: int = 4, n_be: int = 2000, n_gates: int = 3000):
     """The small tree's family scaled up (2,000 events, 3,000 gates in 8
@@ -130,28 +140,110 @@ def uncertainty_tree(seed: int = 4, n_be: int = 2000, n_gates: int = 3000):
     return pd, LognormalModel(pd.var_p.copy(), ef)
 
 
-# The large tree's own top gate is an OR of 11,015 gates, 3,563 of which have
-# probability above 0.9 (complemented rare inputs make 11,492 of its 40,000
-# gates that likely). Its top event held in every one of 2^18 trials at the
-# medians and at four sampled parameter sets, so every Monte Carlo estimate
-# is (trials + 1/2) / (trials + 1) and there is nothing to analyse. Gate
-# 64134 (an OR; 210 gates and 956 basic events below it; about 6e-3 at the
-# medians) has the most basic events below it among the gates of that tree
-# whose probability at the medians lies in [3e-3, 0.15].
-LARGE_TREE_UA_TOP = 64134
+def independence_probabilities(pd: Pdag) -> dict:
+    """Each gate's probability under the independence approximation (inputs
+    treated as independent: AND = product, OR = 1 - product of complements,
+    ATLEAST by the Poisson-binomial distribution of the count, XOR by its
+    parity recursion, NOT/NULL the last input), from the basic events' p.
+    Deterministic and exact for trees without shared inputs; used here only
+    to pick a workload's top structure. -> {pdag index: p}."""
+    from .pdag import AND, ATLEAST, NAND, NOR, NOT, NULL, OR, XOR
+    prob = {int(v): float(p) for v, p in zip(pd.var_index, pd.var_p)}
+    gates = {int(g): s for s, g in enumerate(pd.gate_index)}
+
+    def value(s):
+        ps = [prob[int(a)] if a > 0 else 1.0 - prob[int(-a)] for a in pd.gate_args(s)]
+        t = int(pd.gate_type[s])
+        if t in (AND, NAND):
+            v = float(np.prod(ps))
+        elif t in (OR, NOR):
+            v = 1.0 - float(np.prod([1.0 - q for q in ps]))
+        elif t == XOR:
+            v = 0.0
+            for q in ps:
+                v = v * (1.0 - q) + (1.0 - v) * q
+        elif t in (NOT, NULL):
+            v = ps[-1] if ps else 0.0
+        else:  # ATLEAST: P(count >= k)
+            dist = np.zeros(len(ps) + 1)
+            dist[0] = 1.0
+            for q in ps:
+                dist[1:] = dist[1:] * (1.0 - q) + dist[:-1] * q
+                dist[0] *= 1.0 - q
+            v = float(dist[int(pd.gate_k[s]):].sum())
+        return 1.0 - v if t in (NAND, NOR, NOT) else v
+
+    # Gates in an order where every gate follows its gate inputs.
+    order, state = [], {}
+    for g in gates:
+        stack = [(g, False)]
+        while stack:
+            x, done = stack.pop()
+            if done:
+                if state.get(x) != 2:
+                    state[x] = 2
+                    order.append(x)
+                continue
+            if state.get(x):
+                continue
+            state[x] = 1
+            stack.append((x, True))
+            for a in pd.gate_args(gates[x]):
+                a = abs(int(a))
+                if a in gates and not state.get(a):
+                    stack.append((a, False))
+    for g in order:
+        prob[g] = value(gates[g])
+    return prob
 
 
-def large_tree_uncertainty(top: int = LARGE_TREE_UA_TOP, seed: int = 2):
-    """The large tree (25,000 events, 40,000 gates) with lognormal
-    uncertainty on every basic event (median = the tree's probability,
-    log-uniform in [1e-6, 1e-2]; error factor uniform in [3, 10]), analysed
-    for gate ``top``: -> (that gate's PDAG with the medians as p,
-    LognormalModel of its basic events)."""
-    full = large_tree(seed)
+# The large tree's generator with the top OR over the last generator layer
+# (1,379 gates) is certain: a third of those gates are near-certain, since a
+# complemented rare input makes an OR likely. The other last-layer gates
+# are rare; together their cones hold 24,974 of the 25,000 basic events.
+AT_SCALE_LOW_P = 1e-3
+
+
+def at_scale_tree(seed: int = 2, low_p: float = AT_SCALE_LOW_P) -> Pdag:
+    """The large tree's family (25,000 basic events with p log-uniform in
+    [1e-6, 1e-2]; 40,000 gates in 30 generator layers, fan-in 2-8, 48 % AND /
+    47 % OR / 5 % ATLEAST with n up to 16, 10 % of inputs complemented;
+    ``top="last_layer"``) with a top event that is neither certain nor
+    negligible: the generator's top OR over the last layer becomes
+    AND(A, B), with A the OR of the last-layer gates whose independence-
+    approximation probability at the medians is below ``low_p`` and B the OR
+    of the others (near-certain). Every generated gate stays reachable;
+    the tree has 40,002 gates."""
+    from .pdag import AND, OR
+    full = random_layered_tree(seed, 25_000, 40_000, 30, p_range=(1e-6, 1e-2), fan_in=(2, 8),
+                               mix=(0.48, 0.47, 0.05), atleast_n=(3, 16), neg_frac=0.10,
+                               top="last_layer")
+    prob = independence_probabilities(full)
+    gates = {int(g): s for s, g in enumerate(full.gate_index)}
+    top_args = [int(a) for a in full.gate_args(gates[full.root])]
+    low = [a for a in top_args if prob[a] < low_p]
+    high = [a for a in top_args if prob[a] >= low_p]
+    if len(low) < 2 or len(high) < 2:
+        raise ValueError("the last layer does not split at low_p")
+    variables = list(zip(full.var_index.tolist(), full.var_p.tolist()))
+    body = [(int(g), int(full.gate_type[s]), int(full.gate_k[s]),
+             [int(a) for a in full.gate_args(s)])
+            for s, g in enumerate(full.gate_index) if int(g) != full.root]
+    a_idx, b_idx, top = full.root, full.root + 1, full.root + 2
+    body += [(a_idx, OR, 0, low), (b_idx, OR, 0, high), (top, AND, 0, [a_idx, b_idx])]
+    pd = Pdag.from_lists(variables, body, top)
+    pd.meta = {"seed": seed, "generator": "at_scale_tree", "low_p": low_p,
+               "low_gates": len(low), "high_gates": len(high)}
+    return pd
+
+
+def at_scale_uncertainty(seed: int = 2):
+    """``at_scale_tree`` with lognormal uncertainty on every basic event:
+    median = the tree's probability (log-uniform in [1e-6, 1e-2]), error
+    factor uniform in [3, 10]. -> (pdag with the medians as p, LognormalModel)."""
+    pd = at_scale_tree(seed)
     rng = SplitMix64(seed ^ 0x5EED)
-    ef_all = np.array([3.0 + 7.0 * rng.uniform() for _ in range(full.n_vars)])
-    pd = full.subgraph(top)
-    ef = ef_all[np.isin(full.var_index, pd.var_index)]
+    ef = np.array([3.0 + 7.0 * rng.uniform() for _ in range(pd.n_vars)])
     return pd, LognormalModel(pd.var_p.copy(), ef)
 
 

@@ -102,18 +102,6 @@ def test_subgraph_rejects_a_basic_event():
         pd.subgraph(int(pd.var_index[0]))
 
 
-def test_large_tree_uncertainty_model():
-    pd, model = ua.large_tree_uncertainty()
-    pd.validate()
-    assert (pd.n_vars, pd.n_gates, pd.root) == (956, 210, ua.LARGE_TREE_UA_TOP)
-    assert np.all((model.error_factor >= 3.0) & (model.error_factor < 10.0))
-    assert np.array_equal(model.median, pd.var_p)
-    assert np.all((pd.var_p >= 1e-6) & (pd.var_p <= 1e-2))
-    # Neither certain nor negligible at the medians (2^18 trials on the CPU).
-    ones, _ = orc.run(pd, 1 << 12)
-    assert 3e-3 < int(ones[0]) / float(1 << 18) < 1.5e-2
-
-
 @pytest.mark.gpu
 def test_ua_samples_bit_exact_against_oracle(gpu):
     pd = small_random(31, n_be=40, n_gates=60, n_layers=6)
@@ -181,3 +169,64 @@ def test_surrogate_screening_gains_over_plain_monte_carlo(gpu):
     assert cmp["routed_fraction"] < 0.3, cmp
     # Routed samples carry exactly the plain Monte Carlo value (same keys).
     assert np.array_equal(res.estimates[res.routed], plain[res.routed])
+
+
+def test_chunked_sampling_reproduces_one_draw():
+    m = ua.LognormalModel(np.array([1e-3, 2e-2, 5e-5]), np.array([3.0, 10.0, 5.0]))
+    p, z = m.sample(1000, seed=4)
+    parts = list(m.sample_chunks(1000, seed=4, chunk=333))
+    assert [a for a, _, _ in parts] == [0, 333, 666, 999]
+    assert np.array_equal(np.concatenate([q for _, q, _ in parts]), p)
+    assert np.array_equal(np.concatenate([w for _, _, w in parts]), z)
+
+
+def test_disagreement_bound_is_absolute_in_logit_space():
+    from mcscram_amd import screening as scr
+    # Members 0.06 apart are routed at any magnitude (a rare event's logit
+    # near -4.6 as much as an even one near 0); 0.04 apart never are.
+    for centre in (-9.0, -4.6, 0.0, 2.0):
+        lg = np.array([[centre, centre, centre], [centre + 0.06, centre - 0.04, centre + 0.03]])
+        assert scr.route(lg, 0.05).tolist() == [True, False, False]
+
+
+def test_paired_errors_closed_form():
+    from mcscram_amd import screening as scr
+    mc_vals = np.linspace(0.01, 0.02, 101)
+    e = scr.paired_errors(mc_vals * 1.01, mc_vals)
+    for k in ("mean", "p05", "p50", "p95"):
+        assert e[k]["rel_error"] == pytest.approx(0.01, rel=1e-9)
+    assert e["samples"] == 101
+    assert e["median_abs_logit_error"] == pytest.approx(
+        np.median(np.abs(scr.logit(mc_vals * 1.01) - scr.logit(mc_vals))))
+
+
+def test_independence_probabilities_exact_on_a_tree():
+    # No shared inputs: the independence approximation is exact, so it
+    # matches exhaustive enumeration.
+    from mcscram_amd.pdag import AND, ATLEAST, NAND, NOR, OR, XOR, Pdag
+    from tests.exact import exact_probabilities
+    vs = [(2 + i, 0.05 + 0.07 * i) for i in range(12)]
+    g = [(20, AND, 0, [2, -3]), (21, OR, 0, [4, 5, -6]), (22, ATLEAST, 2, [7, 8, 9]),
+         (23, XOR, 0, [10, -11]), (24, NAND, 0, [20, 13]), (25, NOR, 0, [21, 12]),
+         (26, OR, 0, [22, 23, 24, 25])]
+    pd = Pdag.from_lists(vs, g, 26)
+    got = ua.independence_probabilities(pd)
+    want = exact_probabilities(pd)
+    for idx in (20, 21, 22, 23, 24, 25, 26):
+        assert got[idx] == pytest.approx(want[idx], rel=1e-12), idx
+
+
+def test_at_scale_tree_shape():
+    pd, model = ua.at_scale_uncertainty()
+    assert (pd.n_vars, pd.n_gates) == (25_000, 40_002)
+    pd.validate()
+    gi = {int(g): s for s, g in enumerate(pd.gate_index)}
+    root = gi[pd.root]
+    assert pd.gate_type[root] == mc.pdag.AND and len(pd.gate_args(root)) == 2
+    assert pd.meta["low_gates"] + pd.meta["high_gates"] == 1379
+    # Every generated gate and basic event is reachable from the top.
+    sub = pd.subgraph(pd.root)
+    assert (sub.n_vars, sub.n_gates) == (pd.n_vars, pd.n_gates)
+    prob = ua.independence_probabilities(pd)
+    assert 5e-3 < prob[pd.root] < 5e-2
+    assert np.all((model.error_factor >= 3.0) & (model.error_factor < 10.0))
