@@ -64,6 +64,18 @@ __device__ __forceinline__ uint32_t sload(const uint32_t *p, uint64_t i) {
     return ((const_u32 *)p)[i];
 }
 
+// Descriptors are written before the launch too: scalar loads, one dword
+// at a time (both are four dwords).
+__device__ __forceinline__ GateDesc gate_desc(const GateDesc *p, uint64_t i) {
+    const uint32_t *q = reinterpret_cast<const uint32_t *>(p + i);
+    return GateDesc{sload(q, 0), sload(q, 1), sload(q, 2), sload(q, 3)};
+}
+
+__device__ __forceinline__ EventDesc event_desc(const EventDesc *p, uint64_t i) {
+    const uint32_t *q = reinterpret_cast<const uint32_t *>(p + i);
+    return EventDesc{sload(q, 0), sload(q, 1), sload(q, 2), sload(q, 3)};
+}
+
 // Workgroup-wide OR of two flags (any live 1 bit, any live 0 bit) through
 // LDS: every lane of the workgroup calls it with its own flags and gets the
 // workgroup's summary of the chunk.
@@ -92,9 +104,9 @@ __device__ __forceinline__ void sampler_task(const PassLaunch &L, uint64_t *__re
                                              uint64_t task) {
     const uint32_t e = uint32_t(div_by_reciprocal(task, geo.n_chunks, sf.inv_chunks));
     const uint32_t chunk = uint32_t(task - uint64_t(e) * geo.n_chunks);
-    const EventDesc d = L.events[e];
+    const EventDesc d = event_desc(L.events, e);
     const uint32_t iteration =
-        L.iteration_ptr ? *L.iteration_ptr + L.samp_iteration : L.samp_iteration;
+        L.iteration_ptr ? sload(L.iteration_ptr, 0) + L.samp_iteration : L.samp_iteration;
     const uint32_t c3base = (iteration + 1u) << 6;
     uint64_t *row = words + uint64_t(L.samp_row0 + e) * geo.stride;
     const uint64_t base = uint64_t(chunk) * geo.chunk_words;
@@ -244,7 +256,7 @@ template <bool kFusedArgs>
 __device__ __forceinline__ uint32_t arg_value(uint32_t ai, bool neg, const Task &t) {
     uint32_t v;
     if (kFusedArgs && (ai & kInlineArg)) {
-        const GateDesc c = t.L->tree_gates[ai & ~kInlineArg];
+        const GateDesc c = gate_desc(t.L->tree_gates, ai & ~kInlineArg);
         const Cls f = classify<false>(c.type_k_flags & 0xFFu, (c.type_k_flags >> 8) & 0xFFu,
                                       c.counts & 0xFFFFu, c.counts >> 16,
                                       t.L->arg_rows + c.arg_begin, t);
@@ -340,7 +352,7 @@ __device__ __forceinline__ void fill(Pair (&v)[K], uint64_t w) {
 // which has counted it already.
 template <int K>
 __device__ __forceinline__ void eval_fused(uint32_t ref, const Task &t, Pair (&v)[K], int k0 = 0) {
-    const GateDesc c = t.L->tree_gates[ref & ~kInlineArg];
+    const GateDesc c = gate_desc(t.L->tree_gates, ref & ~kInlineArg);
     const uint32_t *ca = t.L->arg_rows + c.arg_begin;
     const uint32_t n_in = c.counts & 0xFFFFu, n_pos = c.counts >> 16;
     const uint32_t type = c.type_k_flags & 0xFFu;
@@ -388,7 +400,7 @@ __device__ __forceinline__ void tally_constant_fused(const uint32_t *__restrict_
     for (uint32_t i = 0; i < n_in; ++i) {
         const uint32_t ai = sload(a, i);
         if (!(ai & kInlineArg)) continue;
-        const GateDesc c = t.L->tree_gates[ai & ~kInlineArg];
+        const GateDesc c = gate_desc(t.L->tree_gates, ai & ~kInlineArg);
         if (!(c.type_k_flags & kGateTallied)) continue;
         const uint32_t v = arg_value<true>(ai, false, t);
         if (v != kChunkMixed) tally_constant(t, c.out_row, int32_t(v));
@@ -631,7 +643,7 @@ __device__ __forceinline__ void gate_task(const PassLaunch &L, uint64_t *__restr
                                           uint64_t task) {
     const uint32_t chunk = uint32_t(div_by_reciprocal(task, L.n_gates, sf.inv_gates));
     const uint32_t gi = uint32_t(task - uint64_t(chunk) * L.n_gates);
-    const GateDesc d = L.gates[gi];
+    const GateDesc d = gate_desc(L.gates, gi);
     const uint32_t *a = L.arg_rows + d.arg_begin;
     const uint64_t base = uint64_t(chunk) * geo.chunk_words;
     const uint64_t live = geo.n_bp - base < geo.chunk_words ? geo.n_bp - base : geo.chunk_words;
