@@ -198,7 +198,7 @@ def bench_ua(args) -> int:
     ua-scale: the large tree's family (25,000 events, 40,002 gates; top
     AND(A, B) over the last layer, p about 0.02) with lognormal uncertainty
     (error factor uniform in [3, 10]) on every event, all 25,000 the network's
-    inputs; training pairs at 2^20 trials; 100,000 outer samples, routed ones
+    inputs; 16,384 training pairs at 2^20 trials; 100,000 outer samples, routed ones
     (disagreement over 0.05 in logit, plus a random 5 % audit) at 2^24 trials.
     Plain Monte Carlo over all 100,000 would take hours: its rate is the
     routed runs' (same engine, same shape) and the surrogate's error is
@@ -210,7 +210,7 @@ def bench_ua(args) -> int:
         pd, model = ua.at_scale_uncertainty()
         outer = args.ua_outer or 100_000
         words = args.ua_words or 1 << 18
-        train = args.ua_train or 32768
+        train = args.ua_train or 16384
         train_words = args.ua_train_words or 1 << 14
         name = (f"surrogate-screened UA at scale: {pd.n_vars} events / {pd.n_gates} gates "
                 f"(large tree's family, top AND of the last layer's rare and likely gates)")
@@ -229,7 +229,7 @@ def bench_ua(args) -> int:
                                     progress=lambda d, n: say(f"training pairs {d}/{n}"))
     say(f"trained: {info}")
     res = scr.screened_uncertainty(pd, model, ens, outer, words, seed=1234,
-                                   mc_chunk=256 if scale else 1 << 30,
+                                   mc_chunk=1024 if scale else 1 << 30,
                                    progress=lambda d, n: say(f"routed Monte Carlo {d}/{n}"))
     say(f"screened: {res.seconds}, routed {res.routed.mean():.4f}")
     t_scr = res.seconds["total"]
@@ -306,7 +306,7 @@ def main() -> int:
                          "ua-scale: the same on the large tree's family, 100,000 outer "
                          "samples at 2^24 trials")
     ap.add_argument("--ua-train", type=int, default=0,
-                    help="training pairs (ua: 65,536; ua-scale: 32,768)")
+                    help="training pairs (ua: 65,536; ua-scale: 16,384)")
     ap.add_argument("--ua-outer", type=int, default=0,
                     help="outer parameter samples (ua: 16,384; ua-scale: 100,000)")
     ap.add_argument("--ua-words", type=int, default=0,
@@ -449,6 +449,7 @@ def main() -> int:
             "kind": "port",
             "sample": f"oracle, same tree, one pass of {words} words ({words * 64} trials), "
                       f"every gate tallied, {dt:.1f} s on {threads} OpenMP threads",
+            "build": os.path.basename(__import__("oracle.binding", fromlist=["x"]).library_variant()),
             "parity_bit_exact": [int(x) for x in ones] == gpu_ones,
             # Linear in trials: the same oracle over the whole 2^32 trials.
             "extrapolated_seconds_2e32_trials": dt * float(1 << 32) / (words * 64), **host_cpu()}

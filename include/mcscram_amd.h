@@ -350,7 +350,11 @@ typedef struct mcs_batch_config {
     uint32_t max_tile_words;  /* words per tile, power of two <= 64 (0 = 64) */
     uint32_t item_words;      /* largest work item in words (0 = 4096); smaller
                                  when the batch has too few tiles to fill the GPU */
-    uint32_t use_graph;       /* 1: replay the fused launch as a captured hipGraph */
+    uint32_t use_graph;       /* 1: from the second pass on, capture a whole pass (the
+                                 counter kernel, the fused launch, every layered
+                                 pipeline's launches, their fork and join) as a
+                                 hipGraph, one per event-area pattern (two at
+                                 most), and replay it: one graph launch per pass */
     uint32_t key0, key1;      /* Philox key (the reference key by default) */
     uint32_t workgroup_lanes; /* 64..1024, a multiple of 64 (0 = 1024) */
     uint32_t min_fused_tile_words; /* narrowest LDS tile for the fused launch

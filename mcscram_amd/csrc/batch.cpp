@@ -19,8 +19,12 @@
 // has its own stream: after that kernel the pipelines fork from the batch's
 // stream, run beside the fused launch and beside each other (one pipeline's
 // short launches and launch tails leave the CUs to the others), and join it
-// again at the end of the pass. The fused launch can be captured once as a
-// hipGraph and replayed.
+// again at the end of the pass. From the second pass on, a pass is the same
+// launches every time up to the event areas the pipelines alternate
+// between, so with use_graph the whole pass (the counter kernel, the fork,
+// the fused launch, every pipeline's launches and the join) is captured
+// once per area pattern as a hipGraph (two at most) and replayed with one
+// graph launch per pass.
 #include <hip/hip_runtime_api.h>
 
 #include <algorithm>
@@ -107,8 +111,9 @@ uint32_t pick_chunk_words(uint64_t n_bp) {
 struct mcs_batch {
     int device = 0;
     hipStream_t stream = nullptr;
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t graph_exec = nullptr;
+    hipGraph_t pass_graph[2] = {nullptr, nullptr};  // steady passes, by area pattern
+    hipGraphExec_t pass_exec[2] = {nullptr, nullptr};
+    uint64_t pass_key[2] = {~uint64_t(0), ~uint64_t(0)};
     mcs_batch_config cfg{};
     std::vector<TreeHost> trees;
     std::vector<LayeredGroup> groups;
@@ -132,8 +137,10 @@ struct mcs_batch {
         }
         for (LayeredGroup &g : groups)
             if (g.stream) (void)hipStreamSynchronize(g.stream);
-        if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
-        if (graph) (void)hipGraphDestroy(graph);
+        for (int i = 0; i < 2; ++i) {
+            if (pass_exec[i]) (void)hipGraphExecDestroy(pass_exec[i]);
+            if (pass_graph[i]) (void)hipGraphDestroy(pass_graph[i]);
+        }
         for (void *p : allocations) (void)hipFree(p);
         for (LayeredGroup &g : groups) {
             if (g.done) (void)hipEventDestroy(g.done);
@@ -199,18 +206,13 @@ mcs_status enqueue_layered(mcs_batch *b, uint32_t it) {
     return MCS_OK;
 }
 
-mcs_status enqueue_pass(mcs_batch *b) {
-    if (b->iteration == 0xFFFFFFFFu) return fail(MCS_ERR_STATE, "iteration counter exhausted");
-    const uint32_t it = b->iteration + 1;
+// The launches of pass it, and the host bookkeeping they imply.
+mcs_status enqueue_pass_launches(mcs_batch *b, uint32_t it) {
     MCS_HIP(mcs::launch_fused_begin_pass(b->stream, b->d_state));
     // Fork: the pipelines start once the iteration counter has advanced.
     if (!b->groups.empty()) MCS_HIP(hipEventRecord(b->began, b->stream));
     for (LayeredGroup &g : b->groups) MCS_HIP(hipStreamWaitEvent(g.stream, b->began, 0));
-    if (b->graph_exec) {
-        MCS_HIP(hipGraphLaunch(b->graph_exec, b->stream));
-    } else if (b->n_fused) {
-        MCS_HIP(mcs::launch_fused(b->stream, b->launch, b->grid, b->lanes, b->lds_bytes));
-    }
+    if (b->n_fused) MCS_HIP(mcs::launch_fused(b->stream, b->launch, b->grid, b->lanes, b->lds_bytes));
     const mcs_status st = enqueue_layered(b, it);
     if (st != MCS_OK) return st;
     // Join: the batch's stream (and the next pass's first kernel, which
@@ -219,6 +221,66 @@ mcs_status enqueue_pass(mcs_batch *b) {
         MCS_HIP(hipEventRecord(g.done, g.stream));
         MCS_HIP(hipStreamWaitEvent(b->stream, g.done, 0));
     }
+    return MCS_OK;
+}
+
+// Pass it is steady when every pipeline's first tree was sampled by the
+// previous pass: then its launches depend only on the event areas the
+// pipelines start in (the area pattern, one bit per pipeline).
+bool steady_pass(const mcs_batch *b, uint32_t it, uint64_t *key) {
+    if (b->groups.size() > 64) return false;
+    uint64_t k = 0;
+    for (size_t i = 0; i < b->groups.size(); ++i) {
+        const LayeredGroup &g = b->groups[i];
+        if (!(g.primed_pos == 0 && g.primed_iter == it)) return false;
+        k |= (g.seq & 1u) << i;
+    }
+    *key = k;
+    return true;
+}
+
+mcs_status enqueue_pass(mcs_batch *b) {
+    if (b->iteration == 0xFFFFFFFFu) return fail(MCS_ERR_STATE, "iteration counter exhausted");
+    const uint32_t it = b->iteration + 1;
+    uint64_t key = 0;
+    if (!b->cfg.use_graph || !steady_pass(b, it, &key)) {
+        const mcs_status st = enqueue_pass_launches(b, it);
+        if (st != MCS_OK) return st;
+        b->iteration = it;
+        return MCS_OK;
+    }
+    int slot = -1;
+    for (int i = 0; i < 2 && slot < 0; ++i)
+        if (b->pass_exec[i] && b->pass_key[i] == key) slot = i;
+    if (slot >= 0) {
+        // Replay, and advance the host bookkeeping as the launches would.
+        MCS_HIP(hipGraphLaunch(b->pass_exec[slot], b->stream));
+        for (LayeredGroup &g : b->groups) {
+            g.seq += g.trees.size();
+            g.primed_pos = 0;
+            g.primed_iter = it + 1;
+        }
+        b->iteration = it;
+        return MCS_OK;
+    }
+    for (int i = 0; i < 2 && slot < 0; ++i)
+        if (!b->pass_exec[i]) slot = i;
+    if (slot < 0) return fail(MCS_ERR_STATE, "more than two area patterns in steady passes");
+    // Capture this pass (its host bookkeeping advances as for an eager pass),
+    // then launch it.
+    MCS_HIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+    const mcs_status st = enqueue_pass_launches(b, it);
+    hipGraph_t graph = nullptr;
+    const hipError_t e = hipStreamEndCapture(b->stream, &graph);
+    if (st != MCS_OK) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return st;
+    }
+    MCS_HIP(e);
+    b->pass_graph[slot] = graph;
+    MCS_HIP(hipGraphInstantiate(&b->pass_exec[slot], graph, nullptr, nullptr, 0));
+    b->pass_key[slot] = key;
+    MCS_HIP(hipGraphLaunch(b->pass_exec[slot], b->stream));
     b->iteration = it;
     return MCS_OK;
 }
@@ -616,15 +678,6 @@ mcs_status build_batch(const mcs_pdag *pdags, int32_t n_trees, const mcs_sample_
                            evs.size() * sizeof(mcs::EventDesc) + ones_total * 8 + rows * g.sum_stride;
     }
     MCS_HIP(hipStreamSynchronize(b->stream));
-
-    if (cfg.use_graph && b->n_fused) {
-        MCS_HIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
-        hipError_t e1 = mcs::launch_fused(b->stream, b->launch, b->grid, b->lanes, b->lds_bytes);
-        hipError_t e2 = hipStreamEndCapture(b->stream, &b->graph);
-        MCS_HIP(e1);
-        MCS_HIP(e2);
-        MCS_HIP(hipGraphInstantiate(&b->graph_exec, b->graph, nullptr, nullptr, 0));
-    }
     *out = b.release();
     return MCS_OK;
 }

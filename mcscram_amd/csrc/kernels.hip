@@ -407,34 +407,34 @@ __device__ __forceinline__ void tally_constant_fused(const uint32_t *__restrict_
     }
 }
 
-// AND / OR / XOR families over K pairs per lane, row inputs two at a time
-// (2K 16-byte loads in flight per lane, one bitop3 per 32 bits for both).
-// Inputs whose chunks are constant are skipped: the classification found
-// none that decides the gate, so each is neutral (AND: true, OR: false) or,
-// for XOR, part of the parity the caller starts from. With kFused an
-// argument may reference a fused gate (kInlineArg), whose word the task
-// computes in registers; fused gates never reference fused gates.
-template <int K, int Op, bool kFused>
-__device__ __forceinline__ void fold_simple(Pair (&r)[K], const uint32_t *__restrict__ a,
-                                            uint32_t n_in, uint32_t n_pos, const Task &t,
-                                            int k0) {
-    uint32_t i = 0;
-    while (i < n_in) {
+// AND / OR / XOR families over K pairs per lane: the inputs [i0, i1) of one
+// sign (kNeg: complemented), row inputs two at a time (2K 16-byte loads in
+// flight per lane, one bitop3 per 32 bits for both). Inputs whose chunks
+// are constant are skipped: the classification found none that decides the
+// gate, so each is neutral (AND: true, OR: false) or, for XOR, part of the
+// parity the caller starts from. With kFused an argument may reference a
+// fused gate (kInlineArg), whose word the task computes in registers; fused
+// gates never reference fused gates.
+template <int K, int Op, bool kNeg, bool kFused>
+__device__ __forceinline__ void fold_range(Pair (&r)[K], const uint32_t *__restrict__ a,
+                                           uint32_t i0, uint32_t i1, const Task &t, int k0) {
+    uint32_t i = i0;
+    while (i < i1) {
         const uint32_t ai = uniform(sload(a, i));
         if (!folded<kFused>(ai, t)) { i += 1; continue; }
         if (kFused && (ai & kInlineArg)) {
             Pair v[K];
             eval_fused<K>(ai, t, v);
-            fold1<Op, K>(r, v, i >= n_pos);
+            fold1<Op, K>(r, v, kNeg);
             i += 1;
             continue;
         }
         // Two rows at a time, except inside a fused gate, whose loads run
         // while the consumer's accumulator is live (register pressure).
         uint32_t j = i + 1;
-        while (kFused && j < n_in && !folded<kFused>(uniform(sload(a, j)), t)) ++j;
-        const uint32_t aj = j < n_in ? uniform(sload(a, j)) : kInlineArg;
-        if (kFused && j < n_in && !(aj & kInlineArg)) {
+        while (kFused && j < i1 && !folded<kFused>(uniform(sload(a, j)), t)) ++j;
+        const uint32_t aj = j < i1 ? uniform(sload(a, j)) : kInlineArg;
+        if (kFused && j < i1 && !(aj & kInlineArg)) {
             const __amdgpu_buffer_rsrc_t p0 = chunk_of(t, ai), p1 = chunk_of(t, aj);
             Pair v0[K], v1[K];
 #pragma unroll
@@ -442,13 +442,7 @@ __device__ __forceinline__ void fold_simple(Pair (&r)[K], const uint32_t *__rest
                 v0[k] = load_k(p0, t, k0 + k);
                 v1[k] = load_k(p1, t, k0 + k);
             }
-            const bool n0 = i >= n_pos, n1 = j >= n_pos;
-            if (!n1)
-                fold2<Op, false, false>(r, v0, v1);
-            else if (n0)
-                fold2<Op, true, true>(r, v0, v1);
-            else
-                fold2<Op, false, true>(r, v0, v1);
+            fold2<Op, kNeg, kNeg>(r, v0, v1);
             i = j + 1;
             continue;
         }
@@ -456,9 +450,18 @@ __device__ __forceinline__ void fold_simple(Pair (&r)[K], const uint32_t *__rest
         Pair v[K];
 #pragma unroll
         for (int k = 0; k < K; ++k) v[k] = load_k(p0, t, k0 + k);
-        fold1<Op, K>(r, v, i >= n_pos);
+        fold1<Op, K>(r, v, kNeg);
         i += 1;
     }
+}
+
+// Positive inputs come first (n_pos of them), then the complemented ones.
+template <int K, int Op, bool kFused>
+__device__ __forceinline__ void fold_simple(Pair (&r)[K], const uint32_t *__restrict__ a,
+                                            uint32_t n_in, uint32_t n_pos, const Task &t,
+                                            int k0) {
+    fold_range<K, Op, false, kFused>(r, a, 0, n_pos, t, k0);
+    fold_range<K, Op, true, kFused>(r, a, n_pos, n_in, t, k0);
 }
 
 // At most 15 inputs folded, pair k: four carry-save planes hold the count

@@ -227,11 +227,13 @@ mcs_status mcs_ua_create(const mcs_pdag *pdag, const mcs_sample_shape *shape, in
     // Event descriptors of every sample, in event_idx order.
     std::unordered_map<int32_t, int32_t> slot_of_var;
     for (int32_t i = 0; i < pdag->n_vars; ++i) slot_of_var[pdag->var_index[i]] = i;
+    std::vector<int32_t> var_of_event(u->n_ev);
+    for (uint32_t e = 0; e < u->n_ev; ++e) var_of_event[e] = slot_of_var.at(plan.event_pdag_index[e]);
     std::vector<mcs::EventDesc> ev(size_t(n_samples) * u->n_ev);
     for (int32_t k = 0; k < n_samples; ++k) {
         const double *p = p_samples + size_t(k) * size_t(pdag->n_vars);
         for (uint32_t e = 0; e < u->n_ev; ++e) {
-            const int32_t vi = slot_of_var.at(plan.event_pdag_index[e]);
+            const int32_t vi = var_of_event[e];
             double pe = p[vi];
             if (!(pe == pe)) return fail(MCS_ERR_INVALID_ARGUMENT, "sampled probability is NaN");
             pe = pe > 1.0 ? 1.0 : (pe < 0.0 ? 0.0 : pe);

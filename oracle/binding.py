@@ -22,10 +22,31 @@ def _p(a):
     return a.ctypes.data_as(C.c_void_p)
 
 
+_V4_FLAGS = {"avx512f", "avx512bw", "avx512cd", "avx512dq", "avx512vl"}
+
+
+def host_has_avx512() -> bool:
+    """Whether this host's CPU flags include everything x86-64-v4 needs."""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("flags"):
+                    return _V4_FLAGS <= set(line.split(":", 1)[1].split())
+    except OSError:
+        pass
+    return False
+
+
+def library_variant() -> str:
+    """The oracle build this host runs: the AVX-512 one where it can."""
+    v4 = os.path.join(_HERE, "liboracle_v4.so")
+    return v4 if (os.path.exists(v4) and host_has_avx512()) else os.path.join(_HERE, "liboracle.so")
+
+
 def lib():
     global _LIB
     if _LIB is None:
-        path = os.path.join(_HERE, "liboracle.so")
+        path = library_variant()
         if not os.path.exists(path):
             raise RuntimeError(f"oracle library missing: {path} (run make -C oracle)")
         L = C.CDLL(path)

@@ -41,6 +41,9 @@
 #ifdef _OPENMP
 #include <omp.h>
 #endif
+#if defined(__AVX512F__) && defined(__BMI2__)
+#include <immintrin.h>
+#endif
 
 namespace {
 
@@ -199,9 +202,9 @@ inline uint64_t threshold_of(double p) {
     return static_cast<uint64_t>(p * two_to_32);
 }
 
-inline uint64_t sample_word(uint32_t pdag_idx, uint32_t event_idx, uint32_t sample_idx,
-                            uint32_t iteration, uint64_t threshold, uint32_t key0 = kKey0,
-                            uint32_t key1 = kKey1) {
+inline uint64_t sample_word_scalar(uint32_t pdag_idx, uint32_t event_idx, uint32_t sample_idx,
+                                   uint32_t iteration, uint64_t threshold, uint32_t key0,
+                                   uint32_t key1) {
     uint64_t word = 0;
     for (uint32_t gen = 0; gen < 16; ++gen) {
         uint32_t x[4] = {pdag_idx + 1u, event_idx + 1u, sample_idx + 1u,
@@ -212,6 +215,59 @@ inline uint64_t sample_word(uint32_t pdag_idx, uint32_t event_idx, uint32_t samp
         }
     }
     return word;
+}
+
+#if defined(__AVX512F__) && defined(__BMI2__)
+// The same 16 Philox blocks of a word as the 16 32-bit lanes of one AVX-512
+// register per state word: a 32 x 32 -> 64-bit product is two vpmuludq
+// (even lanes, then odd lanes shifted down), recombined into hi and lo
+// halves; bit 4g + j of the word is lane g of the compare mask of output j.
+inline void mul_hilo16(__m512i a, __m512i m, __m512i &hi, __m512i &lo) {
+    const __m512i even = _mm512_mul_epu32(a, m);
+    const __m512i odd = _mm512_mul_epu32(_mm512_srli_epi64(a, 32), m);
+    lo = _mm512_mask_blend_epi32(0xAAAA, even, _mm512_slli_epi64(odd, 32));
+    hi = _mm512_mask_blend_epi32(0xAAAA, _mm512_srli_epi64(even, 32), odd);
+}
+
+inline uint64_t sample_word_avx512(uint32_t pdag_idx, uint32_t event_idx, uint32_t sample_idx,
+                                   uint32_t iteration, uint64_t threshold, uint32_t key0,
+                                   uint32_t key1) {
+    if (threshold == 0) return 0;
+    if (threshold > 0xFFFFFFFFull) return ~uint64_t(0);
+    __m512i x0 = _mm512_set1_epi32(int(pdag_idx + 1u));
+    __m512i x1 = _mm512_set1_epi32(int(event_idx + 1u));
+    __m512i x2 = _mm512_set1_epi32(int(sample_idx + 1u));
+    __m512i x3 = _mm512_add_epi32(_mm512_set1_epi32(int((iteration + 1u) << 6)),
+                                  _mm512_setr_epi32(0, 1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12,
+                                                    13, 14, 15));
+    const __m512i m0 = _mm512_set1_epi32(int(0xD2511F53u)), m1 = _mm512_set1_epi32(int(0xCD9E8D57u));
+    uint32_t k0 = key0, k1 = key1;
+    for (int r = 0; r < 10; ++r) {
+        __m512i ha, la, hb, lb;
+        mul_hilo16(x0, m0, ha, la);
+        mul_hilo16(x2, m1, hb, lb);
+        const __m512i n0 = _mm512_xor_si512(_mm512_xor_si512(hb, x1), _mm512_set1_epi32(int(k0)));
+        const __m512i n2 = _mm512_xor_si512(_mm512_xor_si512(ha, x3), _mm512_set1_epi32(int(k1)));
+        x0 = n0; x1 = lb; x2 = n2; x3 = la;
+        k0 += 0x9E3779B9u;
+        k1 += 0xBB67AE85u;
+    }
+    const __m512i t = _mm512_set1_epi32(int(uint32_t(threshold)));
+    const uint64_t b0 = _mm512_cmplt_epu32_mask(x0, t), b1 = _mm512_cmplt_epu32_mask(x1, t);
+    const uint64_t b2 = _mm512_cmplt_epu32_mask(x2, t), b3 = _mm512_cmplt_epu32_mask(x3, t);
+    return _pdep_u64(b0, 0x1111111111111111ull) | _pdep_u64(b1, 0x2222222222222222ull) |
+           _pdep_u64(b2, 0x4444444444444444ull) | _pdep_u64(b3, 0x8888888888888888ull);
+}
+#endif
+
+inline uint64_t sample_word(uint32_t pdag_idx, uint32_t event_idx, uint32_t sample_idx,
+                            uint32_t iteration, uint64_t threshold, uint32_t key0 = kKey0,
+                            uint32_t key1 = kKey1) {
+#if defined(__AVX512F__) && defined(__BMI2__)
+    return sample_word_avx512(pdag_idx, event_idx, sample_idx, iteration, threshold, key0, key1);
+#else
+    return sample_word_scalar(pdag_idx, event_idx, sample_idx, iteration, threshold, key0, key1);
+#endif
 }
 
 // Gate word from its inputs in the reference's order: positive inputs first,

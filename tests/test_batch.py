@@ -162,3 +162,30 @@ def test_batch_of_all_aralia_models_bit_exact(gpu):
         info = b.info()
     assert info["num_fused_trees"] > 0 and info["num_layered_trees"] > 0
     _check_batch(trees, [1000] * len(trees), n_pass=2)
+
+
+@pytest.mark.gpu
+def test_batch_graph_replays_match_eager_passes(gpu):
+    # Whole-pass graphs: the first pass runs eagerly, the second is captured,
+    # later ones replay one of two graphs (pipelines of an odd number of
+    # trees alternate event areas). Passes are split across calls with
+    # collects (eager work) between replays; every collect must equal the
+    # eager engine's, and the last one the oracle's.
+    trees = _mixed_trees()
+    shapes = [1000, 2048, 1000, 2048, 1000, 2048, 1000]
+    tallies = [_all_nodes(t) for t in trees]
+    runs = {}
+    for g in (0, 1):
+        seen = []
+        with mc.Batch(trees, shapes, tallies, use_graph=g, min_fused_tile_words=32,
+                      layered_streams=2) as b:
+            for n in (1, 2, 1, 3):
+                b.pass_(n)
+                seen.append([[t.num_one_bits for t in b.collect(i)] for i in range(len(trees))])
+            info = b.info()
+        runs[g] = seen
+    assert info["iteration"] == 7 and info["num_layered_trees"] > 0
+    assert runs[0] == runs[1]
+    for i, (pd, n_bp) in enumerate(zip(trees, shapes)):
+        want, _ = orc.run(pd, n_bp, n_iters=7, tally=tallies[i])
+        assert runs[1][-1][i] == [int(x) for x in want], i

@@ -221,3 +221,24 @@ def test_point_estimates_closed_form():
                                      0.25 + 2.5758293035489004 * se], rtol=0, atol=1e-15)
     assert orc.point_estimates(0, 64)[4] == 0.0  # clamped at 0
     assert orc.point_estimates(64, 64)[5] == 1.0  # clamped at 1
+
+
+def test_avx512_oracle_build_equals_the_generic_one():
+    # liboracle_v4.so draws the 16 generations of a word as vector lanes;
+    # its words must equal the generic build's bit for bit, thresholds 0
+    # and 2^32 included.
+    import ctypes as C
+    import os
+    if not (orc.host_has_avx512() and os.path.exists(os.path.join(orc._HERE, "liboracle_v4.so"))):
+        pytest.skip("no AVX-512 host or no liboracle_v4.so")
+    libs = [C.CDLL(os.path.join(orc._HERE, n)) for n in ("liboracle.so", "liboracle_v4.so")]
+    for L in libs:
+        L.orc_sample_word.argtypes = [C.c_uint32] * 4 + [C.c_uint64]
+        L.orc_sample_word.restype = C.c_uint64
+    rng = np.random.default_rng(17)
+    thresholds = [0, 1, 1 << 20, (1 << 31) + 5, (1 << 32) - 1, 1 << 32]
+    for _ in range(3000):
+        a = [int(v) for v in rng.integers(0, 1 << 32, size=4, dtype=np.uint64)]
+        thr = thresholds[int(rng.integers(0, len(thresholds)))] if rng.random() < 0.3 \
+            else int(rng.integers(0, (1 << 32) + 1))
+        assert libs[0].orc_sample_word(*a, thr) == libs[1].orc_sample_word(*a, thr), (a, thr)
