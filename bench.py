@@ -169,7 +169,18 @@ def bench_fused(args, mc) -> int:
     if not args.no_cpu:
         # CPU oracle on a fixed subset of the trees, bit-exact check beside it.
         sub = list(range(0, len(trees), max(1, len(trees) // 8)))[:8]
-        sw = 1 << 10 if args.workload == "batched" else words
+        if args.workload == "batched":
+            # Words per tree doubled from 1,024 while the sample stays near
+            # --cpu-seconds (the oracle's time is linear in words).
+            sw = 1 << 10
+            t = time.perf_counter()
+            for i in sub:
+                orc.run(trees[i], sw, tally=tallies[i])
+            per_word = (time.perf_counter() - t) / sw
+            while sw * 2 <= words and per_word * sw * 2 <= args.cpu_seconds:
+                sw *= 2
+        else:
+            sw = words
         t = time.perf_counter()
         cpu = [orc.run(trees[i], sw, tally=tallies[i])[0] for i in sub]
         dt = time.perf_counter() - t
@@ -181,6 +192,7 @@ def bench_fused(args, mc) -> int:
             "value": sum(trees[i].n_gates for i in sub) * sw * 64 / dt, "unit": "gate-evals/s",
             "cores": orc.max_threads(), "kind": "port",
             "sample": f"oracle, trees {sub}, one pass of {sw} words each, {dt:.1f} s",
+            "build": os.path.basename(orc.library_variant()),
             "parity_bit_exact": ok, **host_cpu()}
     b.close()
     print(json.dumps(out), flush=True)
@@ -188,6 +200,11 @@ def bench_fused(args, mc) -> int:
 
 
 def bench_ua(args) -> int:
+    print(json.dumps(run_ua(args)), flush=True)
+    return 0
+
+
+def run_ua(args) -> dict:
     """Surrogate-screened uncertainty analysis against plain batched GPU Monte
     Carlo.
 
@@ -198,7 +215,7 @@ def bench_ua(args) -> int:
     ua-scale: the large tree's family (25,000 events, 40,002 gates; top
     AND(A, B) over the last layer, p about 0.02) with lognormal uncertainty
     (error factor uniform in [3, 10]) on every event, all 25,000 the network's
-    inputs; 16,384 training pairs at 2^20 trials; 100,000 outer samples, routed ones
+    inputs; 65,536 training pairs at 2^20 trials; 100,000 outer samples, routed ones
     (disagreement over 0.05 in logit, plus a random 5 % audit) at 2^24 trials.
     Plain Monte Carlo over all 100,000 would take hours: its rate is the
     routed runs' (same engine, same shape) and the surrogate's error is
@@ -210,7 +227,7 @@ def bench_ua(args) -> int:
         pd, model = ua.at_scale_uncertainty()
         outer = args.ua_outer or 100_000
         words = args.ua_words or 1 << 18
-        train = args.ua_train or 16384
+        train = args.ua_train or 65536
         train_words = args.ua_train_words or 1 << 14
         name = (f"surrogate-screened UA at scale: {pd.n_vars} events / {pd.n_gates} gates "
                 f"(large tree's family, top AND of the last layer's rare and likely gates)")
@@ -223,11 +240,23 @@ def bench_ua(args) -> int:
         name = "surrogate-screened UA, 2,000 events / 3,000 gates"
     epochs = args.ua_epochs or (60 if scale else 30)
     say = lambda m: print(f"[ua] {m}", file=sys.stderr, flush=True)  # noqa: E731
-    say(f"training pairs: {train} x {train_words * 64} trials")
-    ens, info = scr.train_surrogate(pd, model, train, train_words, seed=3, epochs=epochs,
-                                    chunk=4096 if scale else 16384,
-                                    progress=lambda d, n: say(f"training pairs {d}/{n}"))
-    say(f"trained: {info}")
+    if args.ua_load:
+        # A surrogate trained and saved by an earlier run of this workload
+        # (--ua-save): its training cost is the one that run recorded.
+        from mcscram_amd.surrogate import Ensemble
+        ens, info = Ensemble.load(args.ua_load, input_map=scr.lognormal_input_map(model))
+        say(f"loaded {args.ua_load}: {info}")
+    else:
+        say(f"training pairs: {train} x {train_words * 64} trials")
+        ens, info = scr.train_surrogate(pd, model, train, train_words, seed=3, epochs=epochs,
+                                        chunk=4096 if scale else 16384,
+                                        progress=lambda d, n: say(f"training pairs {d}/{n}"))
+        say(f"trained: {info}")
+        if args.ua_save:
+            ens.save(args.ua_save, info)
+        if args.ua_train_only:
+            return {"workload": name, "phase": "trained", "training": info,
+                    "saved": args.ua_save or None}
     res = scr.screened_uncertainty(pd, model, ens, outer, words, seed=1234,
                                    mc_chunk=1024 if scale else 1 << 30,
                                    progress=lambda d, n: say(f"routed Monte Carlo {d}/{n}"))
@@ -273,8 +302,7 @@ def bench_ua(args) -> int:
     out["speedup_vs_plain_mc"] = t_plain / t_scr
     out["speedup_including_training"] = t_plain / (t_scr + out["training_seconds"])
     out["within_2pct"] = bool(cmp["mean"]["rel_error"] <= 0.02 and cmp["p95"]["rel_error"] <= 0.02)
-    print(json.dumps(out), flush=True)
-    return 0
+    return out
 
 
 def tuned_batch_config(workload: str) -> dict:
@@ -315,9 +343,19 @@ def main() -> int:
                     help="words per training Monte Carlo run (ua: 2^16; ua-scale: 2^14)")
     ap.add_argument("--ua-epochs", type=int, default=0,
                     help="training epochs (ua: 30; ua-scale: 60)")
+    ap.add_argument("--ua-save", default="",
+                    help="write the trained surrogate ensemble to this .npz file")
+    ap.add_argument("--ua-load", default="",
+                    help="screen with an ensemble written by --ua-save instead of training")
+    ap.add_argument("--ua-train-only", action="store_true",
+                    help="stop after training (with --ua-save: the first half of a run "
+                         "split in two)")
     ap.add_argument("--trees", type=int, default=256)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-ua", action="store_true",
+                    help="skip the screened uncertainty analysis (2,000 events / 3,000 gates) "
+                         "that the default layered run also measures")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run a 1 GiB device-to-device copy (reference for HBM counters)")
     args = ap.parse_args()
@@ -455,6 +493,20 @@ def main() -> int:
             "extrapolated_seconds_2e32_trials": dt * float(1 << 32) / (words * 64), **host_cpu()}
     if args.copy_calibration:
         out["copy_calibration_bytes"] = copy_calibration(1 << 30)
+    if not args.no_ua and args.tree == "large":
+        # The metric's UA half in the same run: the 2,000-event / 3,000-gate
+        # workload at its specified size, after the timed passes above. A
+        # failure here is reported in the line instead of losing it.
+        ua_args = argparse.Namespace(**{**vars(args), "workload": "ua"})
+        try:
+            u = run_ua(ua_args)
+            out["ua"] = {k: u[k] for k in ("metric", "value", "unit", "plain_mc_samples_per_s",
+                                           "speedup_vs_plain_mc", "within_2pct", "routed_fraction",
+                                           "disagreed_fraction", "comparison", "audit_paired",
+                                           "training_seconds", "speedup_including_training",
+                                           "config")}
+        except Exception as e:  # noqa: BLE001
+            out["ua"] = {"error": f"{type(e).__name__}: {e}"}
     print(json.dumps(out), flush=True)
     return 0
 

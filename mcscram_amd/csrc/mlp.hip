@@ -61,14 +61,34 @@ struct GemmArgs {
     float *C;           // element (m, n) at C[m * c_sm + n]
     int64_t c_sm;
     int32_t M, N, K;    // M, N multiples of 64, K a multiple of 32
+    // Split-K: blockIdx.z covers k-steps [z * steps, (z + 1) * steps) and,
+    // with more than one split, writes its 64 x 64 partial to
+    // partial[z][m][n] (M x N floats each) for gemm_reduce; steps = all
+    // k-steps and partial unused otherwise.
+    int32_t steps;
+    float *partial;
 };
 
 constexpr int kTileM = 64, kTileN = 64, kTileK = 32, kPadK = kTileK + 8;
 
+// 8 consecutive k of one row of A (or one column of B) as one 16-byte word:
+// a single load when the k stride is 1, eight 2-byte loads otherwise.
+__device__ __forceinline__ uint4 load8(const uint16_t *p, int64_t stride) {
+    if (stride == 1) return *reinterpret_cast<const uint4 *>(p);
+    uint16_t v[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) v[j] = p[int64_t(j) * stride];
+    uint4 r;
+    __builtin_memcpy(&r, v, 16);
+    return r;
+}
+
 // C = A x B. Workgroup: 64 x 64 outputs, four waves of 32 x 32, each a 2 x 2
 // grid of 16 x 16 MFMA tiles; K in steps of 32 through LDS. A is staged as
 // [m][k] and B transposed as [n][k], so both operand fragments (8 BF16 of
-// one row and consecutive k) are single 16-byte LDS reads.
+// one row and consecutive k) are single 16-byte LDS reads. The next step's
+// operands are loaded into registers while the matrix cores work on the
+// current one.
 __global__ __launch_bounds__(256) void gemm_bf16(GemmArgs g) {
     __shared__ __attribute__((aligned(16))) uint16_t As[kTileM][kPadK];
     __shared__ __attribute__((aligned(16))) uint16_t Bt[kTileN][kPadK];
@@ -84,22 +104,22 @@ __global__ __launch_bounds__(256) void gemm_bf16(GemmArgs g) {
         for (int j = 0; j < 2; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
     // Each thread stages 8 consecutive k of one row of A and of one column of B.
     const int sm = tid >> 2, sk = (tid & 3) * 8;
-    for (int k0 = 0; k0 < g.K; k0 += kTileK) {
-        const uint16_t *pa = g.A + int64_t(m0 + sm) * g.a_sm + int64_t(k0 + sk) * g.a_sk;
-        if (g.a_sk == 1) {
-            *reinterpret_cast<uint4 *>(&As[sm][sk]) = *reinterpret_cast<const uint4 *>(pa);
-        } else {
-#pragma unroll
-            for (int j = 0; j < 8; ++j) As[sm][sk + j] = pa[int64_t(j) * g.a_sk];
-        }
-        const uint16_t *pb = g.B + int64_t(k0 + sk) * g.b_sk + int64_t(n0 + sm) * g.b_sn;
-        if (g.b_sk == 1) {
-            *reinterpret_cast<uint4 *>(&Bt[sm][sk]) = *reinterpret_cast<const uint4 *>(pb);
-        } else {
-#pragma unroll
-            for (int j = 0; j < 8; ++j) Bt[sm][sk + j] = pb[int64_t(j) * g.b_sk];
-        }
+    const int k_begin = int(blockIdx.z) * g.steps * kTileK;
+    const int k_end = min(g.K, k_begin + g.steps * kTileK);
+    const uint16_t *pa = g.A + int64_t(m0 + sm) * g.a_sm + int64_t(k_begin + sk) * g.a_sk;
+    const uint16_t *pb = g.B + int64_t(k_begin + sk) * g.b_sk + int64_t(n0 + sm) * g.b_sn;
+    const int64_t a_step = int64_t(kTileK) * g.a_sk, b_step = int64_t(kTileK) * g.b_sk;
+    uint4 ra = load8(pa, g.a_sk), rb = load8(pb, g.b_sk);
+    for (int k0 = k_begin; k0 < k_end; k0 += kTileK) {
+        *reinterpret_cast<uint4 *>(&As[sm][sk]) = ra;
+        *reinterpret_cast<uint4 *>(&Bt[sm][sk]) = rb;
         __syncthreads();
+        if (k0 + kTileK < k_end) {
+            pa += a_step;
+            pb += b_step;
+            ra = load8(pa, g.a_sk);
+            rb = load8(pb, g.b_sk);
+        }
         bf16x8 af[2], bfr[2];
 #pragma unroll
         for (int i = 0; i < 2; ++i) af[i] = *reinterpret_cast<const bf16x8 *>(&As[wm + i * 16 + r][q * 8]);
@@ -114,6 +134,12 @@ __global__ __launch_bounds__(256) void gemm_bf16(GemmArgs g) {
     }
     // Accumulator layout of a 16 x 16 tile: lane holds column (lane % 16),
     // rows 4 * (lane / 16) + 0..3.
+    float *out = g.C;
+    int64_t out_sm = g.c_sm;
+    if (gridDim.z > 1) {
+        out = g.partial + int64_t(blockIdx.z) * g.M * g.N;
+        out_sm = g.N;
+    }
 #pragma unroll
     for (int i = 0; i < 2; ++i)
 #pragma unroll
@@ -121,8 +147,20 @@ __global__ __launch_bounds__(256) void gemm_bf16(GemmArgs g) {
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
                 const int row = m0 + wm + i * 16 + q * 4 + v, col = n0 + wn + j * 16 + r;
-                g.C[int64_t(row) * g.c_sm + col] = acc[i][j][v];
+                out[int64_t(row) * out_sm + col] = acc[i][j][v];
             }
+}
+
+// C = the sum of the split-K partials, in split order (deterministic).
+__global__ void gemm_reduce(const float *__restrict__ partial, int32_t splits, int32_t M, int32_t N,
+                            float *__restrict__ C, int64_t c_sm) {
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    const int64_t mn = int64_t(M) * N;
+    if (i >= mn) return;
+    float s = 0.f;
+    for (int32_t z = 0; z < splits; ++z) s += partial[int64_t(z) * mn + i];
+    const int64_t row = i / N, col = i - row * N;
+    C[row * c_sm + col] = s;
 }
 
 // Rows of x (FP32, n_in wide) gathered by index into BF16 rows Dp wide,
@@ -272,10 +310,32 @@ uint32_t blocks_for(int64_t n, int t = 256) { return uint32_t((n + t - 1) / t); 
 
 }  // namespace
 
-hipError_t launch_gemm_bf16(hipStream_t s, const GemmArgs &g) {
+// Split-K workspace: enough for 512 workgroups' 64 x 64 partials.
+constexpr int64_t kSplitWorkspaceFloats = int64_t(512) * kTileM * kTileN;
+
+// With fewer output tiles than twice the CUs, K is split so that about 512
+// workgroups run, each over at least four k-steps, when a workspace of
+// kSplitWorkspaceFloats floats is given.
+hipError_t launch_gemm_bf16(hipStream_t s, GemmArgs g, float *workspace = nullptr) {
     if (g.M % kTileM || g.N % kTileN || g.K % kTileK || g.M <= 0 || g.N <= 0 || g.K <= 0)
         return hipErrorInvalidValue;
-    hipLaunchKernelGGL(gemm_bf16, dim3(g.N / kTileN, g.M / kTileM), dim3(256), 0, s, g);
+    const int64_t tiles = int64_t(g.M / kTileM) * (g.N / kTileN);
+    const int32_t ksteps = g.K / kTileK;
+    int32_t splits = 1;
+    if (workspace && tiles < 512) {
+        splits = int32_t(std::min<int64_t>(512 / tiles, ksteps / 4));
+        splits = std::max(splits, 1);
+        if (int64_t(splits) * g.M * g.N > kSplitWorkspaceFloats) splits = 1;
+    }
+    g.steps = (ksteps + splits - 1) / splits;
+    splits = (ksteps + g.steps - 1) / g.steps;
+    g.partial = workspace;
+    hipLaunchKernelGGL(gemm_bf16, dim3(g.N / kTileN, g.M / kTileM, splits), dim3(256), 0, s, g);
+    if (splits > 1) {
+        const int64_t mn = int64_t(g.M) * g.N;
+        hipLaunchKernelGGL(gemm_reduce, dim3(uint32_t((mn + 255) / 256)), dim3(256), 0, s,
+                           static_cast<const float *>(workspace), splits, g.M, g.N, g.C, g.c_sm);
+    }
     return hipGetLastError();
 }
 
@@ -303,6 +363,7 @@ struct mcs_mlp {
     float *z1 = nullptr, *z2 = nullptr, *z3 = nullptr, *da = nullptr, *yb = nullptr, *yhat = nullptr,
           *d = nullptr, *sq = nullptr, *scal = nullptr;
     int32_t *idx = nullptr;
+    float *splitws = nullptr;  // split-K partials (mcs::kSplitWorkspaceFloats)
     std::vector<void *> ws;
 
     ~mcs_mlp() {
@@ -347,6 +408,7 @@ mcs_status ensure_workspace(mcs_mlp *n, int32_t rows) {
     MCS_HIP(alloc(&n->sq, R * 4));
     MCS_HIP(alloc(&n->scal, 64 * 4));
     MCS_HIP(alloc(&n->idx, R * 4));
+    MCS_HIP(alloc(&n->splitws, size_t(mcs::kSplitWorkspaceFloats) * 4));
     n->cap = rows;
     return MCS_OK;
 }
@@ -362,13 +424,13 @@ mcs_status forward(mcs_mlp *n, int32_t rows, bool with_loss) {
     const int32_t H = n->h;
     const int64_t RH = int64_t(rows) * H;
     const uint16_t *W1 = n->pb + n->off_w1, *W2 = n->pb + n->off_w2, *W3 = n->pb + n->off_w3;
-    MCS_HIP(mcs::launch_gemm_bf16(s, gemm(n->xb, n->dp, 1, W1, H, 1, n->z1, H, rows, H, n->dp)));
+    MCS_HIP(mcs::launch_gemm_bf16(s, gemm(n->xb, n->dp, 1, W1, H, 1, n->z1, H, rows, H, n->dp), n->splitws));
     hipLaunchKernelGGL(mcs::bias_gelu, dim3(mcs::blocks_for(RH)), dim3(256), 0, s, n->z1,
                        n->p + n->off_b1, H, RH, n->a1);
-    MCS_HIP(mcs::launch_gemm_bf16(s, gemm(n->a1, H, 1, W2, H, 1, n->z2, H, rows, H, H)));
+    MCS_HIP(mcs::launch_gemm_bf16(s, gemm(n->a1, H, 1, W2, H, 1, n->z2, H, rows, H, H), n->splitws));
     hipLaunchKernelGGL(mcs::bias_gelu, dim3(mcs::blocks_for(RH)), dim3(256), 0, s, n->z2,
                        n->p + n->off_b2, H, RH, n->a2);
-    MCS_HIP(mcs::launch_gemm_bf16(s, gemm(n->a2, H, 1, W3, H, 1, n->z3, H, rows, H, H)));
+    MCS_HIP(mcs::launch_gemm_bf16(s, gemm(n->a2, H, 1, W3, H, 1, n->z3, H, rows, H, H), n->splitws));
     hipLaunchKernelGGL(mcs::bias_gelu, dim3(mcs::blocks_for(RH)), dim3(256), 0, s, n->z3,
                        n->p + n->off_b3, H, RH, n->a3);
     hipLaunchKernelGGL(mcs::out_layer, dim3(rows), dim3(64), 0, s, n->a3, n->pb + n->off_w4,
@@ -391,21 +453,21 @@ mcs_status backward(mcs_mlp *n, int32_t rows) {
     hipLaunchKernelGGL(mcs::out_back, dim3(mcs::blocks_for(RH)), dim3(256), 0, s, n->d,
                        n->pb + n->off_w4, n->z3, H, RH, n->dz);
     // Layer 3: gW3 = a2^T dz3, gb3, da2 = dz3 W3^T.
-    MCS_HIP(mcs::launch_gemm_bf16(s, gemm(n->a2, 1, H, n->dz, H, 1, g + n->off_w3, H, H, H, rows)));
+    MCS_HIP(mcs::launch_gemm_bf16(s, gemm(n->a2, 1, H, n->dz, H, 1, g + n->off_w3, H, H, H, rows), n->splitws));
     hipLaunchKernelGGL(mcs::col_sums, dim3(mcs::blocks_for(H, 64)), dim3(64 * mcs::kColWaves), 0, s,
                        n->dz, nullptr, rows, H, g + n->off_b3);
-    MCS_HIP(mcs::launch_gemm_bf16(s, gemm(n->dz, H, 1, W3, 1, H, n->da, H, rows, H, H)));
+    MCS_HIP(mcs::launch_gemm_bf16(s, gemm(n->dz, H, 1, W3, 1, H, n->da, H, rows, H, H), n->splitws));
     hipLaunchKernelGGL(mcs::gelu_back, dim3(mcs::blocks_for(RH)), dim3(256), 0, s, n->da, n->z2, RH,
                        n->dz);
     // Layer 2.
-    MCS_HIP(mcs::launch_gemm_bf16(s, gemm(n->a1, 1, H, n->dz, H, 1, g + n->off_w2, H, H, H, rows)));
+    MCS_HIP(mcs::launch_gemm_bf16(s, gemm(n->a1, 1, H, n->dz, H, 1, g + n->off_w2, H, H, H, rows), n->splitws));
     hipLaunchKernelGGL(mcs::col_sums, dim3(mcs::blocks_for(H, 64)), dim3(64 * mcs::kColWaves), 0, s,
                        n->dz, nullptr, rows, H, g + n->off_b2);
-    MCS_HIP(mcs::launch_gemm_bf16(s, gemm(n->dz, H, 1, W2, 1, H, n->da, H, rows, H, H)));
+    MCS_HIP(mcs::launch_gemm_bf16(s, gemm(n->dz, H, 1, W2, 1, H, n->da, H, rows, H, H), n->splitws));
     hipLaunchKernelGGL(mcs::gelu_back, dim3(mcs::blocks_for(RH)), dim3(256), 0, s, n->da, n->z1, RH,
                        n->dz);
     // Layer 1: gW1 = x^T dz1 (dp x h), gb1.
-    MCS_HIP(mcs::launch_gemm_bf16(s, gemm(n->xb, 1, n->dp, n->dz, H, 1, g + n->off_w1, H, n->dp, H, rows)));
+    MCS_HIP(mcs::launch_gemm_bf16(s, gemm(n->xb, 1, n->dp, n->dz, H, 1, g + n->off_w1, H, n->dp, H, rows), n->splitws));
     hipLaunchKernelGGL(mcs::col_sums, dim3(mcs::blocks_for(H, 64)), dim3(64 * mcs::kColWaves), 0, s,
                        n->dz, nullptr, rows, H, g + n->off_b1);
     MCS_HIP(hipGetLastError());
@@ -740,7 +802,7 @@ mcs_status mcs_mlp_gemm_test(const float *a, const float *b, int32_t M, int32_t 
     for (size_t i = 0; i < ha.size(); ++i) ha[i] = mcs::to_bf16(a[i]);
     for (size_t i = 0; i < hb.size(); ++i) hb[i] = mcs::to_bf16(b[i]);
     uint16_t *da = nullptr, *db = nullptr;
-    float *dc = nullptr;
+    float *dc = nullptr, *dw = nullptr;
     MCS_HIP(hipMalloc(&da, ha.size() * 2));
     hipError_t e = hipMalloc(&db, hb.size() * 2);
     if (e == hipSuccess) e = hipMalloc(&dc, size_t(M) * N * 4);
@@ -750,11 +812,15 @@ mcs_status mcs_mlp_gemm_test(const float *a, const float *b, int32_t M, int32_t 
         // a is [M x K] row-major, or [K x M] row-major when trans_a.
         const int64_t a_sm = trans_a ? 1 : K, a_sk = trans_a ? M : 1;
         const int64_t b_sk = trans_b ? 1 : N, b_sn = trans_b ? K : 1;
-        e = mcs::launch_gemm_bf16(nullptr, gemm(da, a_sm, a_sk, db, b_sk, b_sn, dc, N, M, N, K));
+        e = hipMalloc(&dw, size_t(mcs::kSplitWorkspaceFloats) * 4);
+        if (e == hipSuccess)
+            e = mcs::launch_gemm_bf16(nullptr, gemm(da, a_sm, a_sk, db, b_sk, b_sn, dc, N, M, N, K),
+                                      dw);
     }
     if (e == hipSuccess) e = hipDeviceSynchronize();
     if (e == hipSuccess) e = hipMemcpy(c, dc, size_t(M) * N * 4, hipMemcpyDeviceToHost);
-    for (void *q : {static_cast<void *>(da), static_cast<void *>(db), static_cast<void *>(dc)})
+    for (void *q : {static_cast<void *>(da), static_cast<void *>(db), static_cast<void *>(dc),
+                    static_cast<void *>(dw)})
         if (q) (void)hipFree(q);
     MCS_HIP(e);
     return MCS_OK;

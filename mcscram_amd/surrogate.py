@@ -149,3 +149,57 @@ class Ensemble:
     def predict(self, x) -> np.ndarray:
         """-> [members, rows] logits."""
         return np.stack([m.predict(x).astype(np.float64) * self.sd + self.mu for m in self.members])
+
+    def save(self, path: str, info: dict | None = None) -> None:
+        """Write what inference uses: each member's weight matrices as their
+        BF16 values (the network multiplies in BF16), biases in FP32, the
+        target standardisation and ``info`` (JSON), to an .npz file."""
+        import json
+        arrs = {"mu": np.float64(self.mu), "sd": np.float64(self.sd),
+                "shape": np.array([self.members[0].n_in, self.members[0].hidden,
+                                   len(self.members)], dtype=np.int64),
+                "info": np.array(json.dumps(info or {}))}
+        for i, m in enumerate(self.members):
+            p = m.params()
+            for name, (a, b) in _param_ranges(m.n_in, m.hidden).items():
+                v = p[a:b]
+                arrs[f"m{i}_{name}"] = _to_bf16_bits(v) if name[0] in "Ww" else v
+        np.savez(path, **arrs)
+
+    @classmethod
+    def load(cls, path: str, input_map=None):
+        """An ensemble written by ``save``. -> (ensemble, info dict)."""
+        import json
+        z = np.load(path, allow_pickle=False)
+        n_in, hidden, members = (int(v) for v in z["shape"])
+        ens = cls(n_in, hidden, members, input_map=input_map)
+        ens.mu, ens.sd = float(z["mu"]), float(z["sd"])
+        for i, m in enumerate(ens.members):
+            p = np.zeros(m.n_params, dtype=np.float32)
+            for name, (a, b) in _param_ranges(n_in, hidden).items():
+                v = z[f"m{i}_{name}"]
+                p[a:b] = _from_bf16_bits(v) if name[0] in "Ww" else v
+            m.set_params(p)
+        return ens, json.loads(str(z["info"]))
+
+
+def _param_ranges(n_in: int, h: int) -> dict:
+    """Slices of the parameter vector: W1 [n_in x h], b1, W2 [h x h], b2, W3,
+    b3, w4 [h], b4."""
+    sizes = [("W1", n_in * h), ("b1", h), ("W2", h * h), ("b2", h), ("W3", h * h), ("b3", h),
+             ("w4", h), ("b4", 1)]
+    out, o = {}, 0
+    for name, n in sizes:
+        out[name] = (o, o + n)
+        o += n
+    return out
+
+
+def _to_bf16_bits(x) -> np.ndarray:
+    """FP32 -> BF16 bit patterns, round to nearest even (finite values)."""
+    u = np.ascontiguousarray(x, dtype=np.float32).view(np.uint32).astype(np.uint64)
+    return ((u + 0x7FFF + ((u >> 16) & 1)) >> 16).astype(np.uint16)
+
+
+def _from_bf16_bits(b) -> np.ndarray:
+    return (np.asarray(b, dtype=np.uint32) << 16).view(np.float32)

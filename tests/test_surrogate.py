@@ -57,7 +57,12 @@ def test_oracle_prediction_consistent_with_loss():
 @pytest.mark.gpu
 @pytest.mark.parametrize("shape,ta,tb", [((64, 64, 32), False, False), ((128, 192, 96), False, False),
                                           ((64, 128, 64), True, False), ((128, 64, 64), False, True),
-                                          ((192, 64, 256), True, True)])
+                                          ((192, 64, 256), True, True),
+                                          # split-K: few output tiles, long K (even and
+                                          # uneven splits, the at-scale first layer)
+                                          ((64, 512, 4096), False, False),
+                                          ((128, 64, 1184), False, True),
+                                          ((256, 512, 25024), False, False)])
 def test_mfma_gemm_against_numpy(gpu, shape, ta, tb):
     from mcscram_amd.surrogate import gemm_test
     M, N, K = shape
@@ -166,3 +171,29 @@ def test_input_map_prediction_and_gradient(gpu):
     assert np.allclose(net.predict(x), ref.predict(p, x, n_in, h), rtol=1e-2, atol=1e-3)
     with pytest.raises(Exception):
         net.set_input_map(scale, shift, None)
+
+
+@pytest.mark.gpu
+def test_ensemble_save_and_load_reproduce_predictions(gpu, tmp_path):
+    from mcscram_amd.surrogate import Ensemble
+    rng = np.random.default_rng(21)
+    x = rng.standard_normal((512, 70)).astype(np.float32)
+    y = x[:, :5].sum(axis=1)
+    scale = (0.5 + rng.random(70)).astype(np.float32)
+    shift = (-rng.random(70)).astype(np.float32)
+    cap = np.full(70, 4.0, dtype=np.float32)
+    ens = Ensemble(70, hidden=64, members=2, seed=4, input_map=(scale, shift, cap))
+    ens.fit(x, y, epochs=3, batch=128)
+    path = str(tmp_path / "ens.npz")
+    ens.save(path, {"pairs": 512})
+    back, info = Ensemble.load(path, input_map=(scale, shift, cap))
+    assert info == {"pairs": 512}
+    # Inference multiplies BF16 weights: saving their BF16 values loses nothing.
+    assert np.array_equal(back.predict(x), ens.predict(x))
+
+
+def test_bf16_bit_patterns_round_like_the_oracle():
+    from mcscram_amd.surrogate import _from_bf16_bits, _to_bf16_bits
+    x = np.random.default_rng(3).standard_normal(200_000).astype(np.float32) * 7.0
+    x[:4] = [0.0, -0.0, 1.0, 3.0e38]
+    assert np.array_equal(_from_bf16_bits(_to_bf16_bits(x)), ref.bf16(x).astype(np.float32))
