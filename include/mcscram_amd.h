@@ -408,15 +408,23 @@ mcs_status mcs_batch_time_passes(mcs_batch *batch, uint32_t n_passes, double *ke
  * per sample; its DirectEval engine ignores the sampled probabilities
  * (src/mc/queue/kernel_builder.h:179-181), so there is no reference Monte
  * Carlo counterpart. Here the samples share the tree's gate descriptors and
- * run as one pipeline: sample k's gate launches sample the basic events of
- * sample k + 1. Sample k draws from Philox key (key0 + sample_id[k], key1),
- * or from (key0, key1) for every sample with common_random_numbers.
+ * run in groups: one launch evaluates a gate layer for every sample of a
+ * group, and a group's gate launches sample the basic events of the next
+ * group. Sample k draws from Philox key (key0 + sample_id[k], key1), or from
+ * (key0, key1) for every sample with common_random_numbers. The grouping
+ * changes no bit: each sample's tallies equal a one-sample run of the same
+ * key.
  */
 typedef struct mcs_ua mcs_ua;
 
 typedef struct mcs_ua_config {
     uint32_t key0, key1;             /* base key (the reference key by default) */
     uint32_t common_random_numbers;  /* 1: every sample draws the same stream */
+    uint32_t samples_per_launch;     /* samples per group (0: about 16,384 tasks per launch,
+                                        at most 64 samples, arenas within 96 GiB) */
+    uint32_t pipelines;              /* pipelines on their own streams (0: up to 2) */
+    uint32_t chunk_words;            /* words per task: 512, 1024, 2048 (0: the longest the
+                                        words per pass fill) */
 } mcs_ua_config;
 
 void mcs_ua_config_default(mcs_ua_config *cfg);
@@ -438,6 +446,11 @@ mcs_status mcs_ua_run(mcs_ua *ua, uint32_t n_passes);
 mcs_status mcs_ua_collect(mcs_ua *ua, uint64_t *ones, int64_t n_out, uint64_t *total_bits);
 /* Device time of n_passes over every sample (measurement hook). */
 mcs_status mcs_ua_time_run(mcs_ua *ua, uint32_t n_passes, double *kernel_ms);
+/* Samples per launch and pipelines chosen at create (fewer than asked when
+ * the grid or device memory requires), words per task, and the device bytes
+ * held. Any output may be null. */
+mcs_status mcs_ua_info(const mcs_ua *ua, uint32_t *samples_per_launch, uint32_t *pipelines,
+                       uint32_t *chunk_words, uint64_t *device_bytes);
 
 /*
  * Surrogate network for uncertainty analysis (no counterpart in the

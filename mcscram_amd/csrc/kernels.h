@@ -96,6 +96,22 @@ struct PassLaunch {
     uint8_t *state;                // chunk summaries [row][state_stride] (null: none,
                                    // every chunk is stored and loaded)
     uint32_t state_stride;         // bytes per row: n_chunks rounded up to 4
+    // Several parameter samples of one tree per launch (uncertainty
+    // analysis). Sample s owns the words at words + s * sample_words, the
+    // summaries at state + s * sample_state and the tally counters at
+    // ones + s * sample_ones. The gate tasks run for samples 0 .. n_samples-1
+    // (sample-major, then chunk-major). With samp_events > 0 the sampler
+    // range indexes (sample, event, chunk) triples, sample-major, with
+    // samp_events events per sample: sample s reads event descriptors
+    // events + s * samp_events and draws with key0 = keys0[s] (key0 when
+    // keys0 is null). n_samples <= 1 and samp_events = 0 is the one-sample
+    // launch of the single-tree engine.
+    uint32_t n_samples;
+    uint32_t samp_events;
+    uint64_t sample_words;
+    uint64_t sample_state;
+    uint64_t sample_ones;
+    const uint32_t *keys0;
 };
 
 // Bytes per row of the chunk-summary array.
@@ -166,7 +182,14 @@ struct SplitFracs {
     uint64_t frac[kXcds];  // interleave_frac of each XCD's share
     uint64_t inv_gates;    // reciprocal64(n_gates)
     uint64_t inv_chunks;   // reciprocal64(n_chunks)
+    uint64_t inv_sample_gate_tasks;  // reciprocal64(n_gates * n_chunks)
+    uint64_t inv_sample_samp_tasks;  // reciprocal64(samp_events * n_chunks)
 };
+
+// Gate tasks of a launch: one per (sample, gate, chunk).
+__host__ __device__ inline uint64_t gate_tasks(const PassLaunch &l, uint32_t n_chunks) {
+    return uint64_t(l.n_gates) * n_chunks * (l.n_samples > 1 ? l.n_samples : 1u);
+}
 
 // Blocks of 256 lanes; the caller checks grid_blocks(...) * 256 < 2^32.
 uint64_t pass_grid_blocks(const PassLaunch &l, const LaunchGeometry &geo);

@@ -665,21 +665,43 @@ __device__ __forceinline__ void gate_task(const PassLaunch &L, uint64_t *__restr
 __global__ __launch_bounds__(256) void pass_layer(PassLaunch L, uint64_t *__restrict__ words,
                                                   unsigned long long *__restrict__ ones,
                                                   LaunchGeometry geo, SplitFracs sf) {
-    const uint64_t ng = uint64_t(L.n_gates) * geo.n_chunks;
+    const uint64_t ng = gate_tasks(L, geo.n_chunks);
     const uint64_t ns = L.samp_end - L.samp_begin;
     const uint32_t x = blockIdx.x % kXcds;
     const uint64_t j = blockIdx.x / kXcds;
     const XcdShare sh = xcd_share(ng, ns, x);
     if (j >= sh.tx) return;
     const uint64_t before = gates_before(j, sh.ngx, sf.frac[x]);
-    if (gates_before(j + 1, sh.ngx, sf.frac[x]) > before)
-        gate_task(L, words, ones, geo, sf, sh.g_lo + before);
+    const bool is_gate = gates_before(j + 1, sh.ngx, sf.frac[x]) > before;
+    uint64_t task = is_gate ? sh.g_lo + before : L.samp_begin + sh.s_lo + (j - before);
+    // The task's parameter sample: its words, summaries, counters, event
+    // descriptors and key (sample 0, and nothing moves, in a one-sample launch).
+    uint64_t s = 0;
+    if (is_gate && L.n_samples > 1) {
+        const uint64_t per = uint64_t(L.n_gates) * geo.n_chunks;
+        s = div_by_reciprocal(task, per, sf.inv_sample_gate_tasks);
+        task -= s * per;
+    } else if (!is_gate && L.samp_events) {
+        const uint64_t per = uint64_t(L.samp_events) * geo.n_chunks;
+        s = div_by_reciprocal(task, per, sf.inv_sample_samp_tasks);
+        task -= s * per;
+    }
+    PassLaunch Ls = L;
+    if (s) {
+        if (Ls.state) Ls.state += s * L.sample_state;
+        Ls.events += s * L.samp_events;
+        words += s * L.sample_words;
+        ones += s * L.sample_ones;
+    }
+    if (!is_gate && L.keys0) Ls.key0 = sload(L.keys0, s);
+    if (is_gate)
+        gate_task(Ls, words, ones, geo, sf, task);
     else
-        sampler_task(L, words, geo, sf, L.samp_begin + sh.s_lo + (j - before));
+        sampler_task(Ls, words, geo, sf, task);
 }
 
 uint64_t pass_grid_blocks(const PassLaunch &l, const LaunchGeometry &geo) {
-    const uint64_t ng = uint64_t(l.n_gates) * geo.n_chunks;
+    const uint64_t ng = gate_tasks(l, geo.n_chunks);
     const uint64_t ns = l.samp_end - l.samp_begin;
     uint64_t m = 0;
     for (uint32_t x = 0; x < kXcds; ++x) {
@@ -694,7 +716,7 @@ hipError_t launch_pass(hipStream_t stream, const PassLaunch &l, uint64_t *words,
     const uint64_t blocks = pass_grid_blocks(l, geo);
     if (blocks == 0) return hipSuccess;
     if (blocks >= (uint64_t(1) << 24)) return hipErrorInvalidConfiguration;
-    const uint64_t ng = uint64_t(l.n_gates) * geo.n_chunks;
+    const uint64_t ng = gate_tasks(l, geo.n_chunks);
     const uint64_t ns = l.samp_end - l.samp_begin;
     SplitFracs sf;
     for (uint32_t x = 0; x < kXcds; ++x) {
@@ -703,6 +725,8 @@ hipError_t launch_pass(hipStream_t stream, const PassLaunch &l, uint64_t *words,
     }
     sf.inv_gates = reciprocal64(l.n_gates);
     sf.inv_chunks = reciprocal64(geo.n_chunks);
+    sf.inv_sample_gate_tasks = reciprocal64(uint64_t(l.n_gates) * geo.n_chunks);
+    sf.inv_sample_samp_tasks = reciprocal64(uint64_t(l.samp_events) * geo.n_chunks);
     hipLaunchKernelGGL(pass_layer, dim3(uint32_t(blocks)), dim3(kLanes), 0, stream, l, words, ones,
                        geo, sf);
     return hipGetLastError();

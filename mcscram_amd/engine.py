@@ -133,6 +133,7 @@ def load_library():
         "mcs_ua_run": [vp, u32],
         "mcs_ua_collect": [vp, vp, C.c_int64, vp],
         "mcs_ua_time_run": [vp, u32, vp],
+        "mcs_ua_info": [vp, vp, vp, vp, vp],
     }
     for name, args in sigs.items():
         fn = getattr(L, name)

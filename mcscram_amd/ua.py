@@ -29,7 +29,8 @@ Z95 = 1.6448536269514722  # standard normal quantile of 0.95
 
 class _UaConfig(C.Structure):
     _fields_ = [("key0", C.c_uint32), ("key1", C.c_uint32),
-                ("common_random_numbers", C.c_uint32)]
+                ("common_random_numbers", C.c_uint32), ("samples_per_launch", C.c_uint32),
+                ("pipelines", C.c_uint32), ("chunk_words", C.c_uint32)]
 
 
 class UncertaintyEngine:
@@ -38,10 +39,14 @@ class UncertaintyEngine:
 
     ``p_samples``: [K, n_vars] probabilities in the order of ``pdag.var_index``.
     ``tally``: PDAG indices counted in every sample (default: the root).
+    ``samples_per_launch``, ``pipelines``, ``chunk_words``: the grouping
+    and the words per task (0: the engine's choice); they change no bit of
+    any sample's tallies.
     """
 
     def __init__(self, pdag: Pdag, shape, p_samples, sample_ids=None, tally=None,
-                 key=MCS_REFERENCE_KEY, common_random_numbers=False):
+                 key=MCS_REFERENCE_KEY, common_random_numbers=False, samples_per_launch=0,
+                 pipelines=0, chunk_words=0):
         L = load_library()
         self._view = _PdagView(pdag)
         self._shape = _shape(shape)
@@ -54,7 +59,8 @@ class UncertaintyEngine:
         self.tally = list(tally) if tally else [pdag.root]
         t = np.ascontiguousarray(self.tally, dtype=np.int32)
         self._t = t
-        cfg = _UaConfig(key[0] & 0xFFFFFFFF, key[1] & 0xFFFFFFFF, 1 if common_random_numbers else 0)
+        cfg = _UaConfig(key[0] & 0xFFFFFFFF, key[1] & 0xFFFFFFFF, 1 if common_random_numbers else 0,
+                        int(samples_per_launch), int(pipelines), int(chunk_words))
         h = C.c_void_p()
         _check(L.mcs_ua_create(C.byref(self._view.struct), C.byref(self._shape), self.n_samples,
                                _p(self.p), _p(ids) if ids is not None else None, _p(t), t.shape[0],
@@ -85,6 +91,14 @@ class UncertaintyEngine:
         ms = C.c_double()
         _check(load_library().mcs_ua_time_run(self._h, n_passes, C.byref(ms)))
         return ms.value
+
+    @property
+    def info(self) -> dict:
+        """Samples per launch and pipelines the engine chose, device bytes."""
+        g, p, c, b = C.c_uint32(), C.c_uint32(), C.c_uint32(), C.c_uint64()
+        _check(load_library().mcs_ua_info(self._h, C.byref(g), C.byref(p), C.byref(c), C.byref(b)))
+        return {"samples_per_launch": g.value, "pipelines": p.value, "chunk_words": c.value,
+                "device_bytes": b.value}
 
     def collect(self):
         """-> (ones [n_samples, n_tally] uint64, total trials per sample)."""

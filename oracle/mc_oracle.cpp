@@ -21,10 +21,14 @@
 //   orc_point_estimates src/mc/stats/ci_utils.h:167-201
 //
 // Parity status of each function, and the check that pins it, is stated in
-// the tests under tests/test_oracle_*.py. The event_idx permutation
-// (orc_layout) replays libstdc++'s std::sort with the reference's comparators
-// over the reference's DFS sequence; no reference binary exists to compare it
-// with, so that permutation is "parity unpinned" beyond the replay.
+// tests/test_oracle.py. The event_idx permutation (orc_layout) is "parity
+// unpinned": it is the reference's DFS gather (layer_manager.tpp:33-60),
+// restated from its text, followed by libstdc++'s std::sort with the
+// reference's comparators. The sort is pinned by a separate restatement of
+// libstdc++'s published introsort (tests/introsort.py); the gather order is
+// checked only against other restatements of the same text, and no
+// reference binary, vector or fixture holds an event order to compare with.
+// Results that depend on it are bit-exact against the restated algorithm.
 //
 // Build: oracle/Makefile (g++ -O3 -fopenmp). Everything is extern "C" with
 // plain pointers so the tests can call it through ctypes.

@@ -9,6 +9,11 @@
   model: parameter samples bit-exact against the oracle run with each
   sample's probabilities and key, and the full-width 2000-512-512-512-1
   surrogate's loss and gradient against the FP32 oracle with BF16 rounding.
+* At-scale uncertainty tree: the large tree's family (25,000 events, 40,002
+  gates) with lognormal uncertainty on every event: parameter samples
+  bit-exact against the oracle, the top event at the medians neither
+  certain nor negligible, and the 25000-512-512-512-1 surrogate's loss and
+  gradient against the FP32 oracle.
 """
 import json
 import os
@@ -101,15 +106,38 @@ def test_uncertainty_tree_samples_bit_exact(gpu):
         assert [int(v) for v in ones[k]] == [int(v) for v in want], k
 
 
-def test_full_width_surrogate_gradient_against_oracle(gpu):
-    """2000-512-512-512-1 with the lognormal input map of the uncertainty
-    tree: loss and per-tensor gradient of one 256-row batch against the FP32
-    oracle with the same BF16 roundings (within 2 % norm-wise: accumulation
-    order differs, so a BF16 rounding of a backpropagated value can differ
-    by one unit)."""
+def test_at_scale_tree_samples_bit_exact(gpu):
+    pd, model = ua.at_scale_uncertainty()
+    assert (pd.n_vars, pd.n_gates) == (25_000, 40_002)
+    p, _ = model.sample(3, seed=13)
+    ids = np.array([5, 99_999, 1 << 24], dtype=np.uint32)
+    tally = [pd.root] + [int(g) for g in pd.gate_index[::997]] + [int(pd.var_index[7])]
+    words = 512
+    with ua.UncertaintyEngine(pd, words, p, sample_ids=ids, tally=tally) as eng:
+        eng.run(1)
+        ones, total = eng.collect()
+    assert total == words * 64
+    for k in range(len(ids)):
+        key = ((KEY[0] + int(ids[k])) & 0xFFFFFFFF, KEY[1])
+        want, _ = orc.run(pd.with_probabilities(p[k]), words, tally=tally, key=key)
+        assert [int(v) for v in ones[k]] == [int(v) for v in want], k
+    # The top event at the medians: 2^24 trials put it well inside (1e-3, 0.5).
+    with mc.LayerManager(pd, 1 << 18) as eng:
+        eng.pass_(1)
+        root = next(t for t in eng.collect_tallies() if t.index == pd.root)
+    assert 1e-3 < root.mean < 0.5, root.mean
+
+
+@pytest.mark.parametrize("tree", ["uncertainty", "at_scale"])
+def test_full_width_surrogate_gradient_against_oracle(gpu, tree):
+    """n_in-512-512-512-1 (n_in = 2,000 or 25,000 basic events) with the
+    lognormal input map of its tree: loss and per-tensor gradient of one
+    256-row batch against the FP32 oracle with the same BF16 roundings
+    (within 2 % norm-wise: accumulation order differs, so a BF16 rounding of
+    a backpropagated value can differ by one unit)."""
     from mcscram_amd.screening import lognormal_input_map
     from mcscram_amd.surrogate import Mlp
-    pd, model = ua.uncertainty_tree()
+    pd, model = ua.uncertainty_tree() if tree == "uncertainty" else ua.at_scale_uncertainty()
     n_in, h = pd.n_vars, 512
     net = Mlp(n_in, h, seed=7)
     scale, shift, cap = lognormal_input_map(model)

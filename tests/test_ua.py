@@ -123,6 +123,42 @@ def test_ua_samples_bit_exact_against_oracle(gpu):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("tree", ["random", "all_connectives"])
+def test_ua_samples_per_launch_change_no_bit(gpu, tree):
+    # Eleven samples in groups of 1, 3, 4, 8 and 11 per launch, on one or
+    # two pipelines (groups of uneven size at each pipeline's end), with
+    # tasks of 512, 1024 and 2048 words (1,500 words per pass: the last
+    # chunk is partial), over two runs: every sample's tallies equal the
+    # oracle run with its own key.
+    if tree == "random":
+        pd = small_random(33, n_be=40, n_gates=60, n_layers=6)
+    else:
+        pd = mc.random_layered_tree(34, 40, 70, 6, all_connectives=True, neg_frac=0.2)
+    rng = np.random.default_rng(5)
+    K = 11
+    P = np.clip(pd.var_p[None, :] * np.exp(rng.standard_normal((K, pd.n_vars))), 0, 1)
+    P[4, :] = 1.0
+    P[7, :] = 0.0
+    tally = [pd.root] + [int(g) for g in pd.gate_index[::9]] + [int(pd.var_index[1])]
+    ids = (1000 + 3 * np.arange(K)).astype(np.uint32)
+    want = [_oracle_sample(pd, P[k], 1500, ((KEY[0] + int(ids[k])) & 0xFFFFFFFF, KEY[1]), 3,
+                           tally) for k in range(K)]
+    for g, pipes, cw in [(1, 1, 512), (1, 2, 1024), (3, 1, 2048), (4, 2, 0), (8, 2, 512),
+                         (11, 1, 1024)]:
+        with ua.UncertaintyEngine(pd, 1500, P, sample_ids=ids, tally=tally,
+                                  samples_per_launch=g, pipelines=pipes, chunk_words=cw) as eng:
+            info = eng.info
+            assert (info["samples_per_launch"], info["pipelines"], info["chunk_words"]) == \
+                (min(g, -(-K // pipes)), pipes, cw or 1024)
+            eng.run(1)
+            eng.run(2)
+            ones, total = eng.collect()
+        assert total == 3 * 1500 * 64
+        for k in range(K):
+            assert [int(v) for v in ones[k]] == [int(v) for v in want[k]], (g, pipes, k)
+
+
+@pytest.mark.gpu
 def test_ua_common_random_numbers(gpu):
     pd = small_random(32, n_be=30, n_gates=40, n_layers=5)
     P = np.stack([pd.var_p, pd.var_p, np.clip(pd.var_p * 2, 0, 1)])
