@@ -334,7 +334,7 @@ def main() -> int:
                          "ua-scale: the same on the large tree's family, 100,000 outer "
                          "samples at 2^24 trials")
     ap.add_argument("--ua-train", type=int, default=0,
-                    help="training pairs (ua: 65,536; ua-scale: 16,384)")
+                    help="training pairs (ua and ua-scale: 65,536)")
     ap.add_argument("--ua-outer", type=int, default=0,
                     help="outer parameter samples (ua: 16,384; ua-scale: 100,000)")
     ap.add_argument("--ua-words", type=int, default=0,
