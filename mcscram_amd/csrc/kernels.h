@@ -49,6 +49,7 @@ inline EventDesc make_event_desc(int32_t pdag_index, uint32_t event_idx, double 
 constexpr uint32_t kGateTallied = 1u << 16;     // popcount into ones[out_row]
 constexpr uint32_t kGateNoStore = 1u << 17;     // tally only (basic-event tallies)
 constexpr uint32_t kGateForceStore = 1u << 18;  // store the words even when constant
+constexpr uint32_t kGateHasFused = 1u << 19;    // some argument is a fused gate (kInlineArg)
 
 // Chunk summaries: one byte per (row, chunk) says whether every live word of
 // the row's chunk is zero, every one is all ones, or neither. Every task

@@ -30,11 +30,6 @@ namespace {
 
 constexpr uint32_t kLanes = 256;
 
-__device__ __forceinline__ uint32_t live_popcount(uint64_t w0, uint64_t w1, uint64_t s,
-                                                  uint64_t n_bp) {
-    return (s < n_bp ? __popcll(w0) : 0u) + (s + 1 < n_bp ? __popcll(w1) : 0u);
-}
-
 // Row words are streamed: a row is read by a handful of gates spread over the
 // whole pass, long after it has left the caches, so loads and stores are
 // non-temporal (measured 2 % faster per pass on the large tree).
@@ -180,14 +175,59 @@ __device__ __forceinline__ Pair load_k(__amdgpu_buffer_rsrc_t r, const Task &t, 
     return {v.x, v.y};
 }
 
-__device__ __forceinline__ uint64_t lane_word(const Task &t, int k, int h) {
-    return t.base + 2u * threadIdx.x + 512u * uint32_t(k) + uint32_t(h);
+// Whether word h of the lane's pair k is below n_bp: its offset in the
+// chunk is 2 lane + 512 k + h, and the chunk's first `live` words are live.
+__device__ __forceinline__ bool lane_word_live(const Task &t, int k, int h) {
+    return 2u * threadIdx.x + 512u * uint32_t(k) + uint32_t(h) < t.live;
 }
 
-// Popcount of the lane's pair k, counting only words below n_bp.
-__device__ __forceinline__ uint32_t pair_popcount(const Task &t, const Pair &p, int k) {
-    if (t.full) return uint32_t(__popcll(p.x)) + uint32_t(__popcll(p.y));
-    return live_popcount(p.x, p.y, lane_word(t, k, 0), t.n_bp);
+// One bit count per 32 bits, accumulated (v_bcnt_u32_b32 adds its second
+// operand).
+__device__ __forceinline__ uint32_t popc_add(uint64_t w, uint32_t acc) {
+    return uint32_t(__popcll(w)) + acc;
+}
+
+// a & b & c, one bitop3 per 32 bits.
+__device__ __forceinline__ uint64_t and3_64(uint64_t a, uint64_t b, uint64_t c) {
+    const uint32_t lo = __builtin_amdgcn_bitop3_b32(uint32_t(a), uint32_t(b), uint32_t(c), 0x80);
+    const uint32_t hi =
+        __builtin_amdgcn_bitop3_b32(uint32_t(a >> 32), uint32_t(b >> 32), uint32_t(c >> 32), 0x80);
+    return uint64_t(hi) << 32 | lo;
+}
+
+// Popcount of the lane's pairs k0 .. k0+K-1 and, with kSummary, the OR and
+// AND of their live words (a chunk summary's inputs). A chunk without
+// padding words (every chunk of a row whose words per pass are a multiple of
+// the chunk, and every chunk but the last otherwise) takes the unmasked
+// path; the branch is on a workgroup-uniform flag, so the common case tests
+// no word for liveness.
+template <int K, bool kSummary>
+__device__ __forceinline__ uint32_t pair_stats(const Task &t, const Pair (&r)[K], int k0,
+                                               uint64_t &any, uint64_t &all) {
+    uint32_t pc = 0;
+    if (t.full) {
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            pc = popc_add(r[k].y, popc_add(r[k].x, pc));
+            if (kSummary) {
+                any |= r[k].x | r[k].y;  // v_or3
+                all = and3_64(all, r[k].x, r[k].y);
+            }
+        }
+    } else {
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            if (lane_word_live(t, k0 + k, 0)) {
+                pc = popc_add(r[k].x, pc);
+                if (kSummary) { any |= r[k].x; all &= r[k].x; }
+            }
+            if (lane_word_live(t, k0 + k, 1)) {
+                pc = popc_add(r[k].y, pc);
+                if (kSummary) { any |= r[k].y; all &= r[k].y; }
+            }
+        }
+    }
+    return pc;
 }
 
 // Summary of the task's chunk of a row (kChunkMixed without summaries).
@@ -216,7 +256,15 @@ struct Cls {
 // comes from its own inputs' summaries (fused gates never hold fused gates,
 // so the classification has two levels).
 template <bool kFusedArgs>
-__device__ __forceinline__ uint32_t arg_value(uint32_t ai, bool neg, const Task &t);
+__device__ __forceinline__ uint32_t arg_value(uint32_t ai, uint32_t neg, const Task &t);
+
+// 1 if input i is complemented (positive inputs come first, n_pos of them),
+// else 0, in integer arithmetic: a comparison's boolean would be a lane
+// mask, which costs a vector select and a read-back per input to turn into
+// a number.
+__device__ __forceinline__ uint32_t negated(uint32_t i, uint32_t n_pos) {
+    return uint32_t(int32_t(n_pos - 1u - i) >> 31) & 1u;
+}
 
 template <bool kFusedArgs>
 __device__ __forceinline__ Cls classify(uint32_t type, uint32_t k8, uint32_t n_in, uint32_t n_pos,
@@ -225,7 +273,7 @@ __device__ __forceinline__ Cls classify(uint32_t type, uint32_t k8, uint32_t n_i
     if (!t.L->state || (type == 2u && n_in >= 256u)) return c;
     uint32_t n0 = 0, n1 = 0, last = kChunkZero;
     for (uint32_t i = 0; i < n_in; ++i) {
-        last = arg_value<kFusedArgs>(sload(a, i), i >= n_pos, t);
+        last = arg_value<kFusedArgs>(sload(a, i), negated(i, n_pos), t);
         n0 += last == 0u;
         n1 += last == 1u;
     }
@@ -253,7 +301,7 @@ __device__ __forceinline__ Cls classify(uint32_t type, uint32_t k8, uint32_t n_i
 }
 
 template <bool kFusedArgs>
-__device__ __forceinline__ uint32_t arg_value(uint32_t ai, bool neg, const Task &t) {
+__device__ __forceinline__ uint32_t arg_value(uint32_t ai, uint32_t neg, const Task &t) {
     uint32_t v;
     if (kFusedArgs && (ai & kInlineArg)) {
         const GateDesc c = gate_desc(t.L->tree_gates, ai & ~kInlineArg);
@@ -264,7 +312,7 @@ __device__ __forceinline__ uint32_t arg_value(uint32_t ai, bool neg, const Task 
     } else {
         v = row_summary(t, ai);
     }
-    return v == kChunkMixed ? v : (v ^ uint32_t(neg));
+    return v == kChunkMixed ? v : (v ^ neg);
 }
 
 // Whether input ai (a row or a fused gate) is folded: false when its chunk
@@ -338,7 +386,7 @@ __device__ __forceinline__ void fold1(Pair (&r)[K], const Pair (&v)[K], bool neg
 template <int K, int Op, bool kFused>
 __device__ __forceinline__ void fold_simple(Pair (&r)[K], const uint32_t *__restrict__ a,
                                             uint32_t n_in, uint32_t n_pos, const Task &t,
-                                            int k0 = 0);
+                                            int k0 = 0, bool fused_args = false);
 
 template <int K>
 __device__ __forceinline__ void fill(Pair (&v)[K], uint64_t w) {
@@ -382,13 +430,14 @@ __device__ __forceinline__ void eval_fused(uint32_t ref, const Task &t, Pair (&v
         fold_simple<K, 1, false>(v, ca, n_in, n_pos, t, k0);
     }
     const bool invert = type == 4u || type == 5u || type == 6u;  // NOT, NAND, NOR
-    uint32_t pc = 0;
+    if (invert) {
 #pragma unroll
-    for (int k = 0; k < K; ++k) {
-        if (invert) { v[k].x = ~v[k].x; v[k].y = ~v[k].y; }
-        pc += pair_popcount(t, v[k], k0 + k);
+        for (int k = 0; k < K; ++k) { v[k].x = ~v[k].x; v[k].y = ~v[k].y; }
     }
-    if (c.type_k_flags & kGateTallied) tally_wave(t.ones, c.out_row, pc);
+    if (c.type_k_flags & kGateTallied) {
+        uint64_t any = 0, all = ~0ull;
+        tally_wave(t.ones, c.out_row, pair_stats<K, false>(t, v, k0, any, all));
+    }
 }
 
 // The tallies of the fused gates among a task's inputs whose chunks are
@@ -408,50 +457,69 @@ __device__ __forceinline__ void tally_constant_fused(const uint32_t *__restrict_
 }
 
 // AND / OR / XOR families over K pairs per lane: the inputs [i0, i1) of one
-// sign (kNeg: complemented), row inputs two at a time (2K 16-byte loads in
-// flight per lane, one bitop3 per 32 bits for both). Inputs whose chunks
-// are constant are skipped: the classification found none that decides the
-// gate, so each is neutral (AND: true, OR: false) or, for XOR, part of the
-// parity the caller starts from. With kFused an argument may reference a
-// fused gate (kInlineArg), whose word the task computes in registers; fused
-// gates never reference fused gates.
+// sign (kNeg: complemented). Inputs whose chunks are constant are skipped:
+// the classification found none that decides the gate, so each is neutral
+// (AND: true, OR: false) or, for XOR, part of the parity the caller starts
+// from. With kFused an argument may reference a fused gate (kInlineArg),
+// whose word the task computes in registers; fused gates never reference
+// fused gates. The folds are associative and commutative bit by bit, so the
+// order in which inputs are folded changes no word.
 template <int K, int Op, bool kNeg, bool kFused>
 __device__ __forceinline__ void fold_range(Pair (&r)[K], const uint32_t *__restrict__ a,
-                                           uint32_t i0, uint32_t i1, const Task &t, int k0) {
-    uint32_t i = i0;
-    while (i < i1) {
-        const uint32_t ai = uniform(sload(a, i));
-        if (!folded<kFused>(ai, t)) { i += 1; continue; }
-        if (kFused && (ai & kInlineArg)) {
+                                           uint32_t i0, uint32_t i1, const Task &t, int k0,
+                                           bool fused_args) {
+    if (!kFused) {
+        // Inside a fused gate, whose consumer's accumulator is live: one row
+        // at a time (register pressure).
+        for (uint32_t i = i0; i < i1; ++i) {
+            const uint32_t ai = uniform(sload(a, i));
+            if (!folded<false>(ai, t)) continue;
+            const __amdgpu_buffer_rsrc_t p0 = chunk_of(t, ai);
             Pair v[K];
-            eval_fused<K>(ai, t, v);
-            fold1<Op, K>(r, v, kNeg);
-            i += 1;
-            continue;
-        }
-        // Two rows at a time, except inside a fused gate, whose loads run
-        // while the consumer's accumulator is live (register pressure).
-        uint32_t j = i + 1;
-        while (kFused && j < i1 && !folded<kFused>(uniform(sload(a, j)), t)) ++j;
-        const uint32_t aj = j < i1 ? uniform(sload(a, j)) : kInlineArg;
-        if (kFused && j < i1 && !(aj & kInlineArg)) {
-            const __amdgpu_buffer_rsrc_t p0 = chunk_of(t, ai), p1 = chunk_of(t, aj);
-            Pair v0[K], v1[K];
 #pragma unroll
-            for (int k = 0; k < K; ++k) {
-                v0[k] = load_k(p0, t, k0 + k);
-                v1[k] = load_k(p1, t, k0 + k);
-            }
-            fold2<Op, kNeg, kNeg>(r, v0, v1);
-            i = j + 1;
+            for (int k = 0; k < K; ++k) v[k] = load_k(p0, t, k0 + k);
+            fold1<Op, K>(r, v, kNeg);
+        }
+        return;
+    }
+    // Rows two at a time (2K 16-byte loads in flight, one bitop3 per 32 bits
+    // for both): a row waits in `held` (a scalar) for the next one, so the
+    // loop body has this one path and the accumulators stay in their
+    // registers from one iteration to the next.
+    uint32_t held = kInlineArg;  // none
+    for (uint32_t i = i0; i < i1; ++i) {
+        const uint32_t ai = uniform(sload(a, i));
+        if ((ai & kInlineArg) || !folded<false>(ai, t)) continue;
+        if (held == kInlineArg) {
+            held = ai;
             continue;
         }
-        const __amdgpu_buffer_rsrc_t p0 = chunk_of(t, ai);
+        const __amdgpu_buffer_rsrc_t p0 = chunk_of(t, held), p1 = chunk_of(t, ai);
+        Pair v0[K], v1[K];
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            v0[k] = load_k(p0, t, k0 + k);
+            v1[k] = load_k(p1, t, k0 + k);
+        }
+        fold2<Op, kNeg, kNeg>(r, v0, v1);
+        held = kInlineArg;
+    }
+    if (held != kInlineArg) {
+        const __amdgpu_buffer_rsrc_t p0 = chunk_of(t, held);
         Pair v[K];
 #pragma unroll
         for (int k = 0; k < K; ++k) v[k] = load_k(p0, t, k0 + k);
         fold1<Op, K>(r, v, kNeg);
-        i += 1;
+    }
+    // Fused inputs (only when the gate has some: kGateHasFused): each one's
+    // word computed here, then folded in.
+    if (!fused_args) return;
+    for (uint32_t i = i0; i < i1; ++i) {
+        const uint32_t ai = uniform(sload(a, i));
+        if (!(ai & kInlineArg) || !folded<true>(ai, t)) continue;
+        Pair v[K];
+        eval_fused<K>(ai, t, v);
+        fold1<Op, K>(r, v, kNeg);
     }
 }
 
@@ -459,19 +527,54 @@ __device__ __forceinline__ void fold_range(Pair (&r)[K], const uint32_t *__restr
 template <int K, int Op, bool kFused>
 __device__ __forceinline__ void fold_simple(Pair (&r)[K], const uint32_t *__restrict__ a,
                                             uint32_t n_in, uint32_t n_pos, const Task &t,
-                                            int k0) {
-    fold_range<K, Op, false, kFused>(r, a, 0, n_pos, t, k0);
-    fold_range<K, Op, true, kFused>(r, a, n_pos, n_in, t, k0);
+                                            int k0, bool fused_args) {
+    fold_range<K, Op, false, kFused>(r, a, 0, n_pos, t, k0, fused_args);
+    fold_range<K, Op, true, kFused>(r, a, n_pos, n_in, t, k0, fused_args);
 }
 
-// At most 15 inputs folded, pair k: four carry-save planes hold the count
-// exactly; a threshold above 15 can never be met (the comparison sees a k
-// bit above the planes as unmet). Inputs with constant chunks are skipped
-// (k8 already counts the constant true ones).
+// Bit-sliced counters: plane b of a counter holds bit b of every trial's
+// count. Adding two input words takes one full adder at plane 0 (sum and
+// majority, one bitop3 each per 32 bits) and a ripple of its carry through
+// the planes above (two operations per plane, one at the top, whose carry
+// out is dropped: the count of at most 2^P - 1 inputs never needs it, and
+// an 8-plane counter wraps as the reference's uint8 count does). Adding
+// inputs one at a time took two operations per plane for each.
+template <int P>
+__device__ __forceinline__ void count_add2(uint64_t (&c)[P], uint64_t a, uint64_t b) {
+    uint64_t carry = bitop3_64<0xE8>(c[0], a, b);  // majority(c0, a, b)
+    c[0] = bitop3_64<0x96>(c[0], a, b);            // c0 ^ a ^ b
+#pragma unroll
+    for (int p = 1; p + 1 < P; ++p) {
+        const uint64_t t = c[p] & carry;
+        c[p] ^= carry;
+        carry = t;
+    }
+    c[P - 1] ^= carry;
+}
+
+template <int P>
+__device__ __forceinline__ void count_add1(uint64_t (&c)[P], uint64_t a) {
+    uint64_t carry = a;
+#pragma unroll
+    for (int p = 0; p + 1 < P; ++p) {
+        const uint64_t t = c[p] & carry;
+        c[p] ^= carry;
+        carry = t;
+    }
+    c[P - 1] ^= carry;
+}
+
+// At most 15 inputs folded, pair k: four planes hold the count exactly; a
+// threshold above 15 can never be met (the comparison sees a k bit above
+// the planes as unmet). Inputs with constant chunks are skipped (k8 already
+// counts the constant true ones). Inputs are added two at a time: one is
+// held until the next arrives.
 template <bool kFused>
 __device__ __forceinline__ Pair atleast_pair4(const uint32_t *__restrict__ a, uint32_t n_in,
                                               uint32_t n_pos, uint32_t k8, const Task &t, int k) {
     uint64_t cx[4] = {0, 0, 0, 0}, cy[4] = {0, 0, 0, 0};
+    Pair held{0, 0};
+    bool holding = false;  // wave-uniform
     for (uint32_t i = 0; i < n_in; ++i) {
         const uint32_t ai = uniform(sload(a, i));
         if (!folded<kFused>(ai, t)) continue;
@@ -484,14 +587,18 @@ __device__ __forceinline__ Pair atleast_pair4(const uint32_t *__restrict__ a, ui
             v = load_k(chunk_of(t, ai), t, k);
         }
         if (i >= n_pos) { v.x = ~v.x; v.y = ~v.y; }
-#pragma unroll
-        for (int b = 0; b < 4; ++b) {
-            const uint64_t tx = cx[b] & v.x, ty = cy[b] & v.y;
-            cx[b] ^= v.x;
-            cy[b] ^= v.y;
-            v.x = tx;
-            v.y = ty;
+        if (!holding) {
+            held = v;
+            holding = true;
+            continue;
         }
+        count_add2<4>(cx, held.x, v.x);
+        count_add2<4>(cy, held.y, v.y);
+        holding = false;
+    }
+    if (holding) {
+        count_add1<4>(cx, held.x);
+        count_add1<4>(cy, held.y);
     }
     if (k8 > 15u) return {0, 0};
     return {atleast_compare<uint64_t, 4>(cx, k8), atleast_compare<uint64_t, 4>(cy, k8)};
@@ -506,6 +613,8 @@ __device__ __forceinline__ uint64_t atleast_word8(const uint32_t *__restrict__ a
                                                   int h) {
     uint64_t c[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     const bool wide = n_in >= 256u;
+    uint64_t held = 0;
+    bool holding = false;  // wave-uniform
     for (uint32_t i = 0; i < n_in; ++i) {
         const uint32_t ai = uniform(sload(a, i));
         const uint32_t st = row_summary(t, ai);
@@ -518,13 +627,15 @@ __device__ __forceinline__ uint64_t atleast_word8(const uint32_t *__restrict__ a
                               int32_t(t.lane_off + 4096u * uint32_t(k) + 8u * uint32_t(h)), 0,
                               kNonTemporal));
         if (i >= n_pos) v = ~v;
-#pragma unroll
-        for (int b = 0; b < 8; ++b) {
-            const uint64_t x = c[b] & v;
-            c[b] ^= v;
-            v = x;
+        if (!holding) {
+            held = v;
+            holding = true;
+            continue;
         }
+        count_add2<8>(c, held, v);
+        holding = false;
     }
+    if (holding) count_add1<8>(c, held);
     return atleast_compare<uint64_t, 8>(c, k8);
 }
 
@@ -538,6 +649,7 @@ __device__ __forceinline__ void eval_gate(const GateDesc &d, const uint32_t *__r
     const uint32_t n_in = d.counts & 0xFFFFu;
     const uint32_t n_pos = d.counts >> 16;
     const uint32_t type = d.type_k_flags & 0xFFu;
+    const bool fused_args = kFused && (d.type_k_flags & kGateHasFused);
     if (type == 2u) {  // ATLEAST
 #pragma unroll
         for (int k = 0; k < K; ++k) {
@@ -561,11 +673,11 @@ __device__ __forceinline__ void eval_gate(const GateDesc &d, const uint32_t *__r
         const bool and_like = (type == 0u || type == 5u);
         fill<K>(r, and_like ? ~0ull : (cls.flip ? ~0ull : 0ull));
         if (and_like)
-            fold_simple<K, 0, kFused>(r, a, n_in, n_pos, t);
+            fold_simple<K, 0, kFused>(r, a, n_in, n_pos, t, 0, fused_args);
         else if (type == 3u)
-            fold_simple<K, 2, kFused>(r, a, n_in, n_pos, t);
+            fold_simple<K, 2, kFused>(r, a, n_in, n_pos, t, 0, fused_args);
         else
-            fold_simple<K, 1, kFused>(r, a, n_in, n_pos, t);
+            fold_simple<K, 1, kFused>(r, a, n_in, n_pos, t, 0, fused_args);
     }
     if (type == 4u || type == 5u || type == 6u) {  // NOT, NAND, NOR
 #pragma unroll
@@ -596,11 +708,12 @@ __device__ __forceinline__ void gate_task_k(const GateDesc &d, const uint32_t *_
     const bool summarise = t.L->state && store;
     const Cls cls = classify<true>(d.type_k_flags & 0xFFu, (d.type_k_flags >> 8) & 0xFFu, n_in,
                                    d.counts >> 16, a, t);
-    tally_constant_fused(a, n_in, t);
+    const bool fused_args = d.type_k_flags & kGateHasFused;
+    if (fused_args) tally_constant_fused(a, n_in, t);
     if (cls.konst >= 0) {
         // Decided by summaries alone: no input is loaded. Fused inputs whose
         // chunks are not constant are still evaluated, for their tallies.
-        for (uint32_t i = 0; i < n_in; ++i) {
+        for (uint32_t i = 0; fused_args && i < n_in; ++i) {
             const uint32_t ai = uniform(sload(a, i));
             if ((ai & kInlineArg) && folded<true>(ai, t)) {
                 Pair v[K];
@@ -618,21 +731,13 @@ __device__ __forceinline__ void gate_task_k(const GateDesc &d, const uint32_t *_
     }
     Pair r[K];
     eval_gate<K, true>(d, a, t, cls, r);
-    if (d.type_k_flags & kGateTallied) {
-        uint32_t pc = 0;
-#pragma unroll
-        for (int k = 0; k < K; ++k) pc += pair_popcount(t, r[k], k);
-        tally_wave(t.ones, out, pc);
-    }
+    uint64_t any = 0, all = ~0ull;
+    const uint32_t pc = summarise ? pair_stats<K, true>(t, r, 0, any, all)
+                                  : pair_stats<K, false>(t, r, 0, any, all);
+    if (d.type_k_flags & kGateTallied) tally_wave(t.ones, out, pc);
     if (!store) return;
     uint32_t st = kChunkMixed;
     if (summarise) {
-        uint64_t any = 0, all = ~0ull;
-#pragma unroll
-        for (int k = 0; k < K; ++k) {
-            if (t.full || lane_word(t, k, 0) < t.n_bp) { any |= r[k].x; all &= r[k].x; }
-            if (t.full || lane_word(t, k, 1) < t.n_bp) { any |= r[k].y; all &= r[k].y; }
-        }
         st = chunk_summary(any != 0, all != ~0ull);
         put_summary(*t.L, out, t.chunk, st);
     }

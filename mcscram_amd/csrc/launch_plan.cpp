@@ -38,6 +38,7 @@ void build_launch_plan(const Plan &plan, const Schedule &sch, const std::vector<
                             const std::vector<ExecGate> &inl) {
             const uint32_t arg_begin = uint32_t(out.args[0].size());
             const uint32_t n_in = uint32_t(pos.size() + neg.size());
+            const size_t kids_before = kids.size();
             for (const auto *v : {&pos, &neg})
                 for (int32_t i : *v) {
                     const ExecGate *ch = nullptr;
@@ -65,6 +66,7 @@ void build_launch_plan(const Plan &plan, const Schedule &sch, const std::vector<
                     out.args[0].push_back(0);
                     out.args[1].push_back(0);
                 }
+            if (kids.size() > kids_before) flags |= kGateHasFused;
             out.gates.push_back(desc(out_row, type, k, flags, arg_begin, n_in,
                                      uint32_t(pos.size())));
             cost += (flags & kGateNoStore) ? 0 : 1;
