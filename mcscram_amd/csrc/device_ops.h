@@ -39,6 +39,19 @@ __device__ __forceinline__ void philox_round(uint32_t &x0, uint32_t &x1, uint32_
     x3 = la;
 }
 
+// acc * 2 + (x < thr): the subtraction's borrow is the comparison, and an
+// add with carry shifts it in, two VALU instructions per draw. (Written as
+// C++, with or without the carry builtins, the compiler emits a compare, a
+// select and a shift-or instead.) thr is wave-uniform.
+__device__ __forceinline__ uint32_t shift_in_below(uint32_t acc, uint32_t x, uint32_t thr) {
+    uint32_t diff;
+    asm("v_sub_co_u32 %1, vcc, %2, %3\n\tv_addc_co_u32 %0, vcc, %0, %0, vcc"
+        : "+v"(acc), "=&v"(diff)
+        : "v"(x), "s"(thr)
+        : "vcc");
+    return acc;
+}
+
 // 64 Bernoulli draws for one word: Philox4x32-10 on counters
 // (c0, c1, c2, c3base + g), g = 0..15, bit 4g+j = (y_j < thr).
 //
@@ -54,8 +67,9 @@ __device__ __forceinline__ void philox_round(uint32_t &x0, uint32_t &x1, uint32_
 //    generations at the small probabilities of fault trees) the compares
 //    and shifts are skipped by one wave-uniform branch.
 //  * kDense = true: no branch; generations run from 15 down to 0 and each
-//    draw's comparison is shifted into the word (compare + shift-add), for
-//    waves where hits are frequent and the branch would always be taken.
+//    draw's comparison is shifted into the word (subtract-with-borrow +
+//    add-with-carry), for waves where hits are frequent and the branch
+//    would always be taken.
 template <bool kDense>
 __device__ __forceinline__ uint64_t bernoulli_word_t(uint32_t c0, uint32_t c1, uint32_t c2,
                                                      uint32_t c3base, uint32_t k0, uint32_t k1,
@@ -104,10 +118,10 @@ __device__ __forceinline__ uint64_t bernoulli_word_t(uint32_t c0, uint32_t c1, u
             uint32_t x0, x1, x2, x3;
             draw(uint32_t(g), x0, x1, x2, x3);
             uint32_t &acc = g >= 8 ? hi_bits : lo_bits;
-            acc = acc * 2u + uint32_t(x3 < thr);
-            acc = acc * 2u + uint32_t(x2 < thr);
-            acc = acc * 2u + uint32_t(x1 < thr);
-            acc = acc * 2u + uint32_t(x0 < thr);
+            acc = shift_in_below(acc, x3, thr);
+            acc = shift_in_below(acc, x2, thr);
+            acc = shift_in_below(acc, x1, thr);
+            acc = shift_in_below(acc, x0, thr);
         }
     } else {
 #pragma unroll
@@ -128,12 +142,14 @@ __device__ __forceinline__ uint64_t bernoulli_word_t(uint32_t c0, uint32_t c1, u
     return uint64_t(hi_bits) << 32 | lo_bits;
 }
 
-// The branch-free packing costs about 43.5 VALU instructions per Philox
-// block, the branching one 35 plus 11 times the chance P that some lane of
-// the wave hits in a generation, P = 1 - (1 - p)^256. They break even near
-// P = 0.78, p = 5.9e-3: thresholds at or above 1.5 x 2^24 take the dense
-// packing.
-constexpr uint32_t kDenseThreshold = (1u << 24) + (1u << 23);
+// VALU instructions per Philox block of a sampler-only launch (rocprofv3
+// SQ_INSTS_VALU, 2,048 events at one p, stores and summaries included): the
+// branch-free packing 41.3 at any p; the branching one 36.4 at p = 1e-5,
+// 38.8 at 1e-3, 40.7 at 2e-3, 42.2 at 3e-3, 44.9 at 6e-3 and 47.2 at 0.5
+// (the chance that some lane of the wave hits in a generation grows as
+// 1 - (1 - p)^256). They break even near p = 2.4e-3: thresholds at or above
+// 0.625 x 2^24 take the branch-free packing.
+constexpr uint32_t kDenseThreshold = (1u << 23) + (1u << 21);
 
 // The Bernoulli word with the packing chosen per wave.
 __device__ __forceinline__ uint64_t bernoulli_word(uint32_t c0, uint32_t c1, uint32_t c2,
