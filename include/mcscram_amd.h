@@ -453,6 +453,21 @@ mcs_status mcs_ua_info(const mcs_ua *ua, uint32_t *samples_per_launch, uint32_t 
                        uint32_t *chunk_words, uint64_t *device_bytes);
 
 /*
+ * Independence approximation of the top event for many parameter samples
+ * (no counterpart in the reference): the closed-form baseline whose residual
+ * the uncertainty-analysis surrogate learns. Each gate's probability with
+ * its inputs treated as independent (AND: product; OR: one minus the
+ * product of complements; XOR: parity recursion; ATLEAST: Poisson-binomial
+ * tail, with min(k, n - k + 1) at most 16; NOT/NULL: the last input;
+ * NAND/NOR/NOT complemented), in FP64 on the GPU, one thread per sample.
+ * p_samples[k * pdag->n_vars + i] is variable i's probability (the order
+ * of pdag->var_index) in sample k; logit_out[k] receives
+ * ln(p_top / (1 - p_top)) with p_top clamped to [1e-300, 1 - 2^-52].
+ */
+mcs_status mcs_independence_logits(const mcs_pdag *pdag, const float *p_samples,
+                                   int64_t n_samples, double *logit_out);
+
+/*
  * Surrogate network for uncertainty analysis (no counterpart in the
  * reference): an MLP n_in -> hidden x 3 (GELU) -> 1 with BF16 weights and
  * activations, FP32 accumulation on the matrix cores, mean-squared-error
@@ -483,6 +498,10 @@ mcs_status mcs_mlp_predict(mcs_mlp *net, const float *x, int64_t n_rows, float *
  * not null, its gradient in parameter order. */
 mcs_status mcs_mlp_loss_grad(mcs_mlp *net, const float *x, const float *y, int64_t n_rows,
                              float *loss_out, float *grad_out);
+/* Decoupled weight decay of the Adam steps (AdamW: each step also takes
+ * lr * weight_decay * w off every parameter w); 0, the default, is plain
+ * Adam. */
+mcs_status mcs_mlp_set_weight_decay(mcs_mlp *net, float weight_decay);
 /* Adam over shuffled minibatches (batch a multiple of 64) with a cosine
  * learning-rate schedule: epoch e uses lr * (1 + cos(pi e / epochs)) / 2.
  * epoch_loss[e] receives the mean training loss of epoch e (may be null). */

@@ -277,10 +277,12 @@ __global__ __launch_bounds__(64 * kColWaves) void col_sums(const uint16_t *__res
     }
 }
 
-// Adam on FP32 master weights (bias-corrected), refreshing the BF16 copy.
+// Adam on FP32 master weights (bias-corrected), with decoupled weight decay
+// wd (AdamW: w -= lr * wd * w beside the Adam step; 0 is plain Adam),
+// refreshing the BF16 copy.
 __global__ void adam(float *__restrict__ p, const float *__restrict__ g, float *__restrict__ m,
                      float *__restrict__ v, uint16_t *__restrict__ pb, int64_t n, float lr,
-                     float b1, float b2, float eps, float c1, float c2) {
+                     float b1, float b2, float eps, float c1, float c2, float wd) {
     const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const float gi = g[i];
@@ -288,7 +290,7 @@ __global__ void adam(float *__restrict__ p, const float *__restrict__ g, float *
     const float vi = b2 * v[i] + (1.f - b2) * gi * gi;
     m[i] = mi;
     v[i] = vi;
-    const float w = p[i] - lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+    const float w = p[i] - lr * ((mi / c1) / (sqrtf(vi / c2) + eps) + wd * p[i]);
     p[i] = w;
     if (pb) pb[i] = to_bf16(w);
 }
@@ -357,6 +359,7 @@ struct mcs_mlp {
     uint16_t *pb = nullptr;  // BF16 copy (same layout)
     float *map = nullptr;    // input map [scale | shift | cap] or null
     int64_t adam_t = 0;
+    float weight_decay = 0.f;  // decoupled (AdamW); 0: plain Adam
     // Workspace for a batch of `cap` rows.
     int32_t cap = 0;
     uint16_t *xb = nullptr, *a1 = nullptr, *a2 = nullptr, *a3 = nullptr, *dz = nullptr;
@@ -479,7 +482,8 @@ mcs_status adam_step(mcs_mlp *n, float lr) {
     const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
     const float c1 = 1.f - std::pow(b1, float(n->adam_t)), c2 = 1.f - std::pow(b2, float(n->adam_t));
     hipLaunchKernelGGL(mcs::adam, dim3(mcs::blocks_for(n->n_dev)), dim3(256), 0, n->stream, n->p,
-                       n->g, n->m, n->v, n->pb, n->n_dev, lr, b1, b2, eps, c1, c2);
+                       n->g, n->m, n->v, n->pb, n->n_dev, lr, b1, b2, eps, c1, c2,
+                       n->weight_decay);
     MCS_HIP(hipGetLastError());
     return MCS_OK;
 }
@@ -618,6 +622,14 @@ mcs_status mcs_mlp_set_params(mcs_mlp *net, const float *in, int64_t n) {
     MCS_HIP(hipMemset(net->m, 0, size_t(net->n_dev) * 4));
     MCS_HIP(hipMemset(net->v, 0, size_t(net->n_dev) * 4));
     return copy_params(net, const_cast<float *>(in), n, false, nullptr);
+}
+
+mcs_status mcs_mlp_set_weight_decay(mcs_mlp *net, float weight_decay) {
+    if (!net) return fail(MCS_ERR_INVALID_ARGUMENT, "null network");
+    if (!(weight_decay >= 0.f) || weight_decay > 1.f)
+        return fail(MCS_ERR_INVALID_ARGUMENT, "weight decay must be in [0, 1]");
+    net->weight_decay = weight_decay;
+    return MCS_OK;
 }
 
 mcs_status mcs_mlp_set_input_map(mcs_mlp *net, const float *scale, const float *shift,

@@ -23,6 +23,7 @@ _SIGS = {
     "mcs_mlp_get_params": [C.c_void_p, C.c_void_p, C.c_int64],
     "mcs_mlp_set_params": [C.c_void_p, C.c_void_p, C.c_int64],
     "mcs_mlp_set_input_map": [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p],
+    "mcs_mlp_set_weight_decay": [C.c_void_p, C.c_float],
     "mcs_mlp_predict": [C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p],
     "mcs_mlp_loss_grad": [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p],
     "mcs_mlp_fit": [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64, C.c_int32, C.c_int32,
@@ -91,6 +92,10 @@ class Mlp:
             raise ValueError(f"input map arrays must have {self.n_in} entries")
         _check(_lib().mcs_mlp_set_input_map(self._h, *(_p(a) for a in arrs)))
 
+    def set_weight_decay(self, weight_decay: float) -> None:
+        """Decoupled weight decay of later fit steps (0: plain Adam)."""
+        _check(_lib().mcs_mlp_set_weight_decay(self._h, float(weight_decay)))
+
     def predict(self, x) -> np.ndarray:
         x = _f32(x)
         out = np.zeros(x.shape[0], dtype=np.float32)
@@ -133,12 +138,20 @@ class Ensemble:
     is set on every member (Mlp.set_input_map)."""
 
     def __init__(self, n_in: int, hidden: int = 512, members: int = 2, seed: int = 11,
-                 input_map=None):
+                 input_map=None, weight_decay: float = 0.0, baseline: str | None = None,
+                 baseline_clip: float = np.inf):
         self.members = [Mlp(n_in, hidden, seed + 101 * i) for i in range(members)]
         self.mu, self.sd = 0.0, 1.0
-        if input_map is not None:
-            for m in self.members:
+        # The closed-form logit the members' output is added to (None: the
+        # members predict the logit itself), clipped to +-baseline_clip; the
+        # caller supplies its values.
+        self.baseline = baseline
+        self.baseline_clip = float(baseline_clip)
+        for m in self.members:
+            if input_map is not None:
                 m.set_input_map(*input_map)
+            if weight_decay:
+                m.set_weight_decay(weight_decay)
 
     def fit(self, x, y, epochs=10, batch=256, lr=1e-3, seed=5):
         y = np.asarray(y, dtype=np.float64)
@@ -147,7 +160,8 @@ class Ensemble:
         return [m.fit(x, yn, epochs, batch, lr, seed + 17 * i) for i, m in enumerate(self.members)]
 
     def predict(self, x) -> np.ndarray:
-        """-> [members, rows] logits."""
+        """-> [members, rows] logits (with a baseline set: the residual to add
+        to the baseline's logit of the same rows)."""
         return np.stack([m.predict(x).astype(np.float64) * self.sd + self.mu for m in self.members])
 
     def save(self, path: str, info: dict | None = None) -> None:
@@ -158,6 +172,8 @@ class Ensemble:
         arrs = {"mu": np.float64(self.mu), "sd": np.float64(self.sd),
                 "shape": np.array([self.members[0].n_in, self.members[0].hidden,
                                    len(self.members)], dtype=np.int64),
+                "baseline": np.array(self.baseline or ""),
+                "baseline_clip": np.float64(self.baseline_clip),
                 "info": np.array(json.dumps(info or {}))}
         for i, m in enumerate(self.members):
             p = m.params()
@@ -172,7 +188,10 @@ class Ensemble:
         import json
         z = np.load(path, allow_pickle=False)
         n_in, hidden, members = (int(v) for v in z["shape"])
-        ens = cls(n_in, hidden, members, input_map=input_map)
+        baseline = str(z["baseline"]) if "baseline" in z.files else ""
+        clip = float(z["baseline_clip"]) if "baseline_clip" in z.files else np.inf
+        ens = cls(n_in, hidden, members, input_map=input_map, baseline=baseline or None,
+                  baseline_clip=clip)
         ens.mu, ens.sd = float(z["mu"]), float(z["sd"])
         for i, m in enumerate(ens.members):
             p = np.zeros(m.n_params, dtype=np.float32)
