@@ -351,7 +351,9 @@ def main() -> int:
                     help="stop after training (with --ua-save: the first half of a run "
                          "split in two)")
     ap.add_argument("--trees", type=int, default=256)
-    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--cpu-seconds", type=float, default=24.0,
+                    help="bound on the CPU baseline's sample (words double while the oracle's "
+                         "time stays under it)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-ua", action="store_true",
                     help="skip the screened uncertainty analysis (2,000 events / 3,000 gates) "
@@ -478,7 +480,7 @@ def main() -> int:
     }
 
     if not args.no_cpu:
-        words, ones, dt, threads = cpu_baseline(pd, args.cpu_seconds, 1 << 15)
+        words, ones, dt, threads = cpu_baseline(pd, args.cpu_seconds, 1 << 17)
         with mc.LayerManager(pd, words, tally=tally) as small:
             small.pass_(1)
             gpu_ones = [t.num_one_bits for t in small.collect_tallies()]
