@@ -468,6 +468,27 @@ mcs_status mcs_independence_logits(const mcs_pdag *pdag, const float *p_samples,
                                    int64_t n_samples, double *logit_out);
 
 /*
+ * Lognormal parameter samples of the basic events, drawn on the GPU. Replaces
+ * the host-side draws of the reference's outer loop (LognormalDeviate,
+ * src/expression/random_deviate.cc:91-108, one call per event and sample in
+ * UncertaintyAnalyzer::Sample, src/uncertainty_analysis.h:154-169, clamped
+ * to [0, 1] as in src/uncertainty_analysis.cc:108-120); the stream is this
+ * engine's own (counter-based, so a row does not depend on how rows are
+ * split over calls), not the reference's std::mt19937 one.
+ * Row r (first_row <= r < first_row + n_rows), variable i:
+ * z = standard normal by Box-Muller from Philox4x32-10 with key
+ * (seed mod 2^32, seed >> 32) and counter (i / 2, r mod 2^32, r >> 32,
+ * 0x4C4E524D): u1 = (top 53 bits of (y0, y1) + 1/2) / 2^53, u2 = top 53 bits
+ * of (y2, y3) / 2^53, z = sqrt(-2 ln u1) cos(2 pi u2) for even i and
+ * sqrt(-2 ln u1) sin(2 pi u2) for odd i; p = min(1, exp(mu_i + sigma_i z))
+ * in FP64. z_out (FP32) and p_out are host arrays
+ * [n_rows * n_vars] (either may be null).
+ */
+mcs_status mcs_lognormal_samples(const double *mu, const double *sigma, int32_t n_vars,
+                                 uint64_t seed, int64_t first_row, int64_t n_rows, float *z_out,
+                                 double *p_out);
+
+/*
  * Surrogate network for uncertainty analysis (no counterpart in the
  * reference): an MLP n_in -> hidden x 3 (GELU) -> 1 with BF16 weights and
  * activations, FP32 accumulation on the matrix cores, mean-squared-error

@@ -37,6 +37,17 @@ SPACE = {
     "workgroup_lanes": [256, 512, 1024],
 }
 STREAMS = [1, 2, 3]  # layered_streams, at full size
+# One small tree alone (320 nodes, 2^20 trials) is a single short launch:
+# what decides its time is how evenly its tiles fill the workgroups and the
+# workgroups the CUs, so narrower tiles, smaller LDS budgets and the work
+# item size are swept instead of the fused/layered split (it always fits).
+SMALL_SPACE = {
+    "max_tile_words": [4, 8, 16, 32],
+    "min_fused_tile_words": [1],
+    "lds_bytes": [16 << 10, 32 << 10, 48 << 10, 64 << 10],
+    "workgroup_lanes": [128, 256, 512, 1024],
+    "item_words": [0, 32, 64],
+}
 
 
 def workload(name: str):
@@ -58,15 +69,20 @@ def time_config(trees, words, cfg, passes=1) -> float:
 
 def sweep(name: str, proxy_div: int = 16, keep: int = 4, log=print) -> dict:
     trees, words = workload(name)
-    proxy_words = max(words // proxy_div, 1024)
+    small = name == "small-fused"
+    # The small tree's pass is a fraction of a millisecond: time it at full
+    # size over many passes instead of on a proxy.
+    proxy_words = words if small else max(words // proxy_div, 1024)
+    passes = 50 if small else 1
+    space = SMALL_SPACE if small else SPACE
     results = []
-    keys = list(SPACE)
-    for values in itertools.product(*(SPACE[k] for k in keys)):
+    keys = list(space)
+    for values in itertools.product(*(space[k] for k in keys)):
         cfg = dict(zip(keys, values))
         if cfg["min_fused_tile_words"] > cfg["max_tile_words"] and cfg["min_fused_tile_words"] != 128:
             continue  # no tree could be fused: same as 128
         try:
-            ms = time_config(trees, proxy_words, cfg)
+            ms = time_config(trees, proxy_words, cfg, passes)
         except Exception as e:  # a shape the device rejects (LDS, occupancy)
             log(f"skip {cfg}: {e}")
             continue
@@ -75,9 +91,9 @@ def sweep(name: str, proxy_div: int = 16, keep: int = 4, log=print) -> dict:
     results.sort(key=lambda r: r["proxy_ms"])
     finals = []
     for r in results[:keep]:
-        for ls in STREAMS:
-            cfg = dict(r["config"], layered_streams=ls)
-            ms = time_config(trees, words, cfg)
+        for ls in ([0] if small else STREAMS):  # no layered tree in the small workload
+            cfg = dict(r["config"], layered_streams=ls) if ls else dict(r["config"])
+            ms = time_config(trees, words, cfg, 4 * passes)
             finals.append({"config": cfg, "ms_per_pass": ms})
             log(f"full  {words} words  {ms:9.3f} ms  {cfg}")
     finals.sort(key=lambda r: r["ms_per_pass"])

@@ -14,7 +14,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libmcscram_amd.so")
-SOURCES = ["kernels.hip", "fused.hip", "mlp.hip", "indep.hip", "engine.cpp", "batch.cpp", "launch_plan.cpp", "ua.cpp", "layering.cpp", "analysis.cpp", "noalloc.hip"]
+SOURCES = ["kernels.hip", "fused.hip", "mlp.hip", "indep.hip", "lognormal.hip", "engine.cpp", "batch.cpp", "launch_plan.cpp", "ua.cpp", "layering.cpp", "analysis.cpp", "noalloc.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 FLAGS = [

@@ -135,6 +135,7 @@ def load_library():
         "mcs_ua_time_run": [vp, u32, vp],
         "mcs_ua_info": [vp, vp, vp, vp, vp],
         "mcs_independence_logits": [vp, vp, C.c_int64, vp],
+        "mcs_lognormal_samples": [vp, vp, i32, u64, C.c_int64, C.c_int64, vp, vp],
     }
     for name, args in sigs.items():
         fn = getattr(L, name)

@@ -1,9 +1,9 @@
 """Surrogate-screened uncertainty analysis.
 
-1. Training pairs: lognormal parameter samples of the tree (their standard-
-   normal scores z, the normalised log probabilities, are the network's
-   input) and the GPU Monte Carlo estimate of the top event for each (its
-   logit is the target).
+1. Training pairs: lognormal parameter samples of the tree, drawn on the GPU
+   (``LognormalModel.draw``; their standard-normal scores z, the normalised
+   log probabilities, are the network's input) and the GPU Monte Carlo
+   estimate of the top event for each (its logit is the target).
 2. A two-member ensemble of surrogate networks fitted to the pairs. Each
    network first maps z_i to p_i / q_i, with q_i the event's 99th percentile
    (``lognormal_input_map``): the top event is a sum of products of event
@@ -122,7 +122,7 @@ def train_surrogate(pd: Pdag, model: LognormalModel, n_train: int, words: int, s
     base = np.zeros(n_train)
     z = np.empty((n_train, pd.n_vars), dtype=np.float32)
     t_mc = t_base = 0.0
-    for a, p, zc in model.sample_chunks(n_train, seed, chunk):
+    for a, p, zc in model.draw_chunks(n_train, seed, chunk):
         b = a + p.shape[0]
         z[a:b] = zc
         p_hat[a:b], dt = monte_carlo(pd, p, words, ids[a:b])
@@ -183,7 +183,7 @@ def screened_uncertainty(pd: Pdag, model: LognormalModel, ens: Ensemble, n_outer
     routed_p = []
     t0 = time.perf_counter()
     t_pred = 0.0
-    for a, p, z in model.sample_chunks(n_outer, seed, chunk):
+    for a, p, z in model.draw_chunks(n_outer, seed, chunk):
         b = a + p.shape[0]
         t = time.perf_counter()
         lg = ens.predict(z)                   # [members, rows]
@@ -225,7 +225,7 @@ def plain_uncertainty(pd: Pdag, model: LognormalModel, n_outer: int, words: int,
     screened_uncertainty with the same seed), in runs of ``chunk`` samples,
     each followed by ``progress(done, n_outer)`` if given.
     -> (estimates, seconds)."""
-    p, _ = model.sample(n_outer, seed)
+    p, _ = model.draw(n_outer, seed)
     ids = (id_base + np.arange(n_outer)).astype(np.uint32)
     est = np.empty(n_outer)
     seconds = 0.0

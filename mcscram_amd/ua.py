@@ -10,9 +10,11 @@ parameter samples at once (``UncertaintyEngine``, the C ABI's mcs_ua_*).
 
 Lognormal deviates follow src/expression/random_deviate.cc:91-108 with the
 median parametrisation used by the workloads: ln p = ln(median) + sigma*Z,
-sigma = ln(EF) / z(level), z(0.95) = 1.6448536... The normal draws come from
-numpy's PCG64 with a fixed seed (the reference's std::mt19937 stream is not
-reproduced: parameter sampling is not part of the bit-exact path).
+sigma = ln(EF) / z(level), z(0.95) = 1.6448536... The workloads draw them on
+the GPU (``LognormalModel.draw``: Philox4x32-10 and Box-Muller, counter-based
+per row); ``LognormalModel.sample`` draws them on the host from numpy's PCG64.
+The reference's std::mt19937 stream is not reproduced: parameter sampling is
+not part of the bit-exact path.
 """
 from __future__ import annotations
 
@@ -139,6 +141,27 @@ class LognormalModel:
             z = rng.standard_normal((min(chunk, n - a), self.median.shape[0]))
             p = np.exp(self.mu[None, :] + self.sigma[None, :] * z)
             yield a, np.clip(p, 0.0, 1.0), z
+
+    def draw(self, n: int, seed: int, first_row: int = 0):
+        """Rows first_row .. first_row + n - 1 of the GPU's counter-based
+        stream (mcs_lognormal_samples: Philox4x32-10 and Box-Muller; a row is
+        the same whichever call draws it). -> (p [n, n_vars] float64 clamped
+        to 1, z [n, n_vars] float32)."""
+        nv = int(self.median.shape[0])
+        mu = np.ascontiguousarray(self.mu, dtype=np.float64)
+        sigma = np.ascontiguousarray(self.sigma, dtype=np.float64)
+        z = np.empty((n, nv), dtype=np.float32)
+        p = np.empty((n, nv), dtype=np.float64)
+        _check(load_library().mcs_lognormal_samples(_p(mu), _p(sigma), nv, int(seed), int(first_row),
+                                                    int(n), _p(z), _p(p)))
+        return p, z
+
+    def draw_chunks(self, n: int, seed: int, chunk: int):
+        """``draw(n, seed)`` in consecutive row blocks of at most ``chunk``
+        rows. Yields (first row, p, z)."""
+        for a in range(0, n, chunk):
+            p, z = self.draw(min(chunk, n - a), seed, a)
+            yield a, p, z
 
 
 def uncertainty_tree(seed: int = 4, n_be: int = 2000, n_gates: int = 3000):
