@@ -213,7 +213,7 @@ def run_ua(args) -> dict:
     training pairs and 16,384 outer samples at 2^22 trials; plain Monte Carlo
     over every outer sample is the yardstick.
     ua-scale: the large tree's family (25,000 events, 40,002 gates; top
-    AND(A, B) over the last layer, p about 0.02) with lognormal uncertainty
+    AND(A, B) over the last layer, p about 0.04) with lognormal uncertainty
     (error factor uniform in [3, 10]) on every event, all 25,000 the network's
     inputs; 65,536 training pairs at 2^20 trials; 100,000 outer samples, routed ones
     (disagreement over 0.05 in logit, plus a random 5 % audit) at 2^24 trials.
