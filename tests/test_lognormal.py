@@ -8,7 +8,8 @@ feed which uniform; the normal distribution of 200,000 draws.
 GPU: the kernel against the oracle (z within one FP32 rounding, p within
 1e-12 relative: both sides evaluate log, sqrt, sin, cos and exp in FP64 with
 their own libraries), rows independent of how calls split them, row ids
-past 2^32, odd variable counts, invalid input.
+past 2^32, odd variable counts; invalid input is refused before any device
+call (a CPU test).
 """
 import json
 import os
@@ -144,9 +145,13 @@ def test_gpu_draws_at_the_uncertainty_workload_size(gpu):
         np.testing.assert_allclose(p[r], pr[0], rtol=1e-12)
 
 
-@pytest.mark.gpu
-def test_gpu_draws_reject_invalid_input(gpu):
+def test_draws_reject_invalid_input_before_touching_a_device():
     from mcscram_amd.engine import EngineError
     bad = ua.LognormalModel(np.array([1e-3, 1e-2]), np.array([3.0, 0.5]))  # EF < 1: sigma < 0
     with pytest.raises(EngineError, match="sigma"):
         bad.draw(4, 1)
+    inf = ua.LognormalModel(np.array([1e-3, 0.0]), np.array([3.0, 3.0]))  # ln 0: mu = -inf
+    with np.errstate(divide="ignore"), pytest.raises(EngineError, match="finite mu"):
+        inf.draw(4, 1)
+    p, z = _model(5).draw(0, 1)  # no rows: nothing to draw
+    assert p.shape == (0, 5) and z.shape == (0, 5)
