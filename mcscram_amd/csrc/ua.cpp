@@ -146,44 +146,65 @@ mcs_status enqueue_run(mcs_ua *u, uint32_t n_passes) {
     // Fork: the pipelines start after what the engine's stream holds.
     MCS_HIP(hipEventRecord(u->began, u->stream));
     for (UaPipe &q : u->pipes) MCS_HIP(hipStreamWaitEvent(q.stream, u->began, 0));
+    // One group's launches on its pipeline's stream: the group's own
+    // sampling unless the previous group's launches already drew it, then
+    // its gate launches, which carry the sampling of the pipeline's next
+    // group (or, for its last group, of its first group's next pass).
+    auto enqueue_group = [&](UaPipe &q, uint32_t k0, uint32_t it) -> mcs_status {
+        const uint32_t gk = std::min(G, q.k_end - k0);
+        const uint32_t area = uint32_t(q.seq & 1u);
+        const mcs::PassLaunch base = group_launch(u, q);
+        if (!(q.primed == int64_t(k0) && q.primed_iter == it)) {
+            mcs::PassLaunch pl = base;
+            set_sampling(u, pl, k0, it, area);
+            pl.samp_end = gk * samp_total;
+            MCS_HIP(mcs::launch_pass(q.stream, pl, q.d_words, u->d_ones, u->geo));
+        }
+        const bool last = k0 + G >= q.k_end;
+        const uint32_t nk0 = last ? q.k_begin : k0 + G;
+        const uint32_t nit = last ? it + 1 : it;
+        const uint32_t ngk = std::min(G, q.k_end - nk0);
+        const bool prefetch = nit != 0;  // iteration counter not wrapping
+        const std::vector<uint64_t> &sp = split_for(ngk);
+        unsigned long long *ones = u->d_ones + uint64_t(k0) * u->slots;  // + pad - 2 n_ev
+        for (size_t l = 0; l + 1 < u->lp.launch_begin.size(); ++l) {
+            mcs::PassLaunch pl = base;
+            pl.gates = u->d_gates + u->lp.launch_begin[l];
+            pl.tree_gates = u->d_gates;
+            pl.arg_rows = u->d_args[area];
+            pl.n_gates = u->lp.launch_begin[l + 1] - u->lp.launch_begin[l];
+            pl.n_samples = gk;
+            if (prefetch) {
+                set_sampling(u, pl, nk0, nit, area ^ 1u);
+                pl.samp_begin = sp[l];
+                pl.samp_end = sp[l + 1];
+            }
+            MCS_HIP(mcs::launch_pass(q.stream, pl, q.d_words, ones, u->geo));
+        }
+        q.primed = prefetch ? int64_t(nk0) : -1;
+        q.primed_iter = nit;
+        ++q.seq;
+        return MCS_OK;
+    };
+    std::vector<uint32_t> next(u->pipes.size());
     for (uint32_t p = 0; p < n_passes; ++p) {
         if (u->iteration == 0xFFFFFFFFu) return fail(MCS_ERR_STATE, "iteration counter exhausted");
         const uint32_t it = u->iteration + 1;
-        for (UaPipe &q : u->pipes) {
-            for (uint32_t k0 = q.k_begin; k0 < q.k_end; k0 += G) {
-                const uint32_t gk = std::min(G, q.k_end - k0);
-                const uint32_t area = uint32_t(q.seq & 1u);
-                const mcs::PassLaunch base = group_launch(u, q);
-                if (!(q.primed == int64_t(k0) && q.primed_iter == it)) {
-                    mcs::PassLaunch pl = base;
-                    set_sampling(u, pl, k0, it, area);
-                    pl.samp_end = gk * samp_total;
-                    MCS_HIP(mcs::launch_pass(q.stream, pl, q.d_words, u->d_ones, u->geo));
-                }
-                const bool last = k0 + G >= q.k_end;
-                const uint32_t nk0 = last ? q.k_begin : k0 + G;
-                const uint32_t nit = last ? it + 1 : it;
-                const uint32_t ngk = std::min(G, q.k_end - nk0);
-                const bool prefetch = nit != 0;  // iteration counter not wrapping
-                const std::vector<uint64_t> &sp = split_for(ngk);
-                unsigned long long *ones = u->d_ones + uint64_t(k0) * u->slots;  // + pad - 2 n_ev
-                for (size_t l = 0; l + 1 < u->lp.launch_begin.size(); ++l) {
-                    mcs::PassLaunch pl = base;
-                    pl.gates = u->d_gates + u->lp.launch_begin[l];
-                    pl.tree_gates = u->d_gates;
-                    pl.arg_rows = u->d_args[area];
-                    pl.n_gates = u->lp.launch_begin[l + 1] - u->lp.launch_begin[l];
-                    pl.n_samples = gk;
-                    if (prefetch) {
-                        set_sampling(u, pl, nk0, nit, area ^ 1u);
-                        pl.samp_begin = sp[l];
-                        pl.samp_end = sp[l + 1];
-                    }
-                    MCS_HIP(mcs::launch_pass(q.stream, pl, q.d_words, ones, u->geo));
-                }
-                q.primed = prefetch ? int64_t(nk0) : -1;
-                q.primed_iter = nit;
-                ++q.seq;
+        // Groups are dealt to the pipelines in turn, one group each, so every
+        // stream is fed from the start: submitting one pipeline's whole run
+        // before the next would block the host on that stream's full queue
+        // while the other pipeline sat idle. Each pipeline's own sequence of
+        // launches is the same either way.
+        for (size_t i = 0; i < u->pipes.size(); ++i) next[i] = u->pipes[i].k_begin;
+        for (bool any = true; any;) {
+            any = false;
+            for (size_t i = 0; i < u->pipes.size(); ++i) {
+                UaPipe &q = u->pipes[i];
+                if (next[i] >= q.k_end) continue;
+                const mcs_status st = enqueue_group(q, next[i], it);
+                if (st != MCS_OK) return st;
+                next[i] += G;
+                any = true;
             }
         }
         u->iteration = it;
