@@ -439,6 +439,15 @@ mcs_status mcs_ua_create(const mcs_pdag *pdag, const mcs_sample_shape *shape, in
                          const int32_t *tally_index, int32_t n_tally, const mcs_ua_config *cfg,
                          mcs_ua **out);
 mcs_status mcs_ua_destroy(mcs_ua *ua);
+/* Replaces the engine's parameter samples with a new set (same layout and
+ * clamping as mcs_ua_create; sample_ids may be null), keeping its device
+ * arenas: the engine then behaves as one created with these samples
+ * (iteration 0, zero counters), bit for bit. Large arenas are costly to
+ * allocate again after a free, so a caller running many sets of samples
+ * through one tree loads each set into one engine. Waits for the engine's
+ * earlier work. After a failure the engine refuses to run until a set loads. */
+mcs_status mcs_ua_set_samples(mcs_ua *ua, int32_t n_samples, const double *p_samples,
+                              const uint32_t *sample_ids);
 /* Every sample advances by n_passes iterations (enqueued, not waited for). */
 mcs_status mcs_ua_run(mcs_ua *ua, uint32_t n_passes);
 /* ones[k * n_tally + j] = one bits of tally j in sample k; *total_bits the

@@ -87,6 +87,9 @@ struct mcs_ua {
     mcs::GateDesc *d_gates = nullptr;
     uint32_t *d_args[2] = {nullptr, nullptr};
     unsigned long long *d_ones = nullptr;  // pad of 2 n_ev, then n_samples x slots
+    int32_t capacity = 0;                // samples d_events, d_keys and d_ones hold
+    int32_t n_vars = 0;                  // variables per row of p_samples
+    std::vector<int32_t> var_of_event;   // p_samples column of each event
     uint64_t device_bytes = 0;
 
     ~mcs_ua() {
@@ -134,7 +137,69 @@ void set_sampling(const mcs_ua *u, mcs::PassLaunch &pl, uint32_t k0, uint32_t it
     pl.samp_row0 = area * u->n_ev;
 }
 
+// Loads a set of parameter samples into an engine whose arenas exist: event
+// descriptors and keys (arrays grown when the set is larger than they
+// hold), zero counters, each pipeline's share of the samples, and the state
+// of a fresh engine (iteration 0, no group sampled, every chunk summary
+// "mixed"). Waits for the engine's work first and for the uploads after.
+mcs_status load_samples(mcs_ua *u, int32_t n_samples, const double *p_samples,
+                        const uint32_t *sample_ids) {
+    // No samples are loaded until this call succeeds: runs are refused.
+    u->n_samples = 0;
+    std::vector<mcs::EventDesc> ev(size_t(n_samples) * u->n_ev);
+    for (int32_t k = 0; k < n_samples; ++k) {
+        const double *p = p_samples + size_t(k) * size_t(u->n_vars);
+        for (uint32_t e = 0; e < u->n_ev; ++e) {
+            double pe = p[u->var_of_event[e]];
+            if (!(pe == pe)) return fail(MCS_ERR_INVALID_ARGUMENT, "sampled probability is NaN");
+            pe = pe > 1.0 ? 1.0 : (pe < 0.0 ? 0.0 : pe);
+            ev[size_t(k) * u->n_ev + e] = mcs::make_event_desc(u->plan.event_pdag_index[e], e, pe);
+        }
+    }
+    std::vector<uint32_t> keys(static_cast<size_t>(n_samples));
+    for (int32_t k = 0; k < n_samples; ++k) {
+        const uint32_t id = sample_ids ? sample_ids[k] : uint32_t(k);
+        keys[size_t(k)] = u->cfg.common_random_numbers ? u->cfg.key0 : u->cfg.key0 + id;
+    }
+    MCS_HIP(hipStreamSynchronize(u->stream));
+    for (UaPipe &q : u->pipes) MCS_HIP(hipStreamSynchronize(q.stream));
+    const uint64_t ones_n = 2ull * u->n_ev + uint64_t(n_samples) * u->slots;
+    if (n_samples > u->capacity) {
+        for (void **q : {reinterpret_cast<void **>(&u->d_events), reinterpret_cast<void **>(&u->d_keys),
+                         reinterpret_cast<void **>(&u->d_ones)})
+            if (*q) {
+                MCS_HIP(hipFree(*q));
+                *q = nullptr;
+            }
+        u->capacity = 0;
+        MCS_HIP(hipMalloc(&u->d_events, ev.size() * sizeof(mcs::EventDesc)));
+        MCS_HIP(hipMalloc(&u->d_keys, keys.size() * 4));
+        MCS_HIP(hipMalloc(&u->d_ones, ones_n * 8));
+        u->capacity = n_samples;
+    }
+    MCS_HIP(hipMemcpyAsync(u->d_events, ev.data(), ev.size() * sizeof(mcs::EventDesc),
+                           hipMemcpyHostToDevice, u->stream));
+    MCS_HIP(hipMemcpyAsync(u->d_keys, keys.data(), keys.size() * 4, hipMemcpyHostToDevice, u->stream));
+    MCS_HIP(hipMemsetAsync(u->d_ones, 0, ones_n * 8, u->stream));
+    const uint64_t sum_bytes = uint64_t(u->group) * u->slot_rows * mcs::state_stride_for(u->geo.n_chunks);
+    const size_t n_pipes = u->pipes.size();
+    for (size_t i = 0; i < n_pipes; ++i) {
+        UaPipe &q = u->pipes[i];
+        q.k_begin = uint32_t(uint64_t(n_samples) * i / n_pipes);
+        q.k_end = uint32_t(uint64_t(n_samples) * (i + 1) / n_pipes);
+        q.seq = 0;
+        q.primed = -1;
+        q.primed_iter = 0;
+        MCS_HIP(hipMemsetAsync(q.d_sum, mcs::kChunkMixed, sum_bytes, u->stream));
+    }
+    MCS_HIP(hipStreamSynchronize(u->stream));
+    u->n_samples = n_samples;
+    u->iteration = 0;
+    return MCS_OK;
+}
+
 mcs_status enqueue_run(mcs_ua *u, uint32_t n_passes) {
+    if (u->n_samples <= 0) return fail(MCS_ERR_STATE, "no parameter samples loaded");
     const uint64_t samp_total = uint64_t(u->n_ev) * u->geo.n_chunks;  // per sample
     const uint32_t G = u->group;
     // Sampler tasks per gate launch, for a group of each size that occurs.
@@ -296,27 +361,18 @@ mcs_status mcs_ua_create(const mcs_pdag *pdag, const mcs_sample_shape *shape, in
     };
     if (!grid_fits(1)) return fail(MCS_ERR_INVALID_ARGUMENT, "grid too large: reduce words per pass");
 
-    // Event descriptors of every sample, in event_idx order.
+    // Event descriptors of every sample, in event_idx order: the column of
+    // p_samples each event reads.
     std::unordered_map<int32_t, int32_t> slot_of_var;
     for (int32_t i = 0; i < pdag->n_vars; ++i) slot_of_var[pdag->var_index[i]] = i;
-    std::vector<int32_t> var_of_event(u->n_ev);
-    for (uint32_t e = 0; e < u->n_ev; ++e) var_of_event[e] = slot_of_var.at(plan.event_pdag_index[e]);
-    std::vector<mcs::EventDesc> ev(size_t(n_samples) * u->n_ev);
-    for (int32_t k = 0; k < n_samples; ++k) {
-        const double *p = p_samples + size_t(k) * size_t(pdag->n_vars);
+    u->n_vars = pdag->n_vars;
+    u->var_of_event.resize(u->n_ev);
+    for (uint32_t e = 0; e < u->n_ev; ++e) u->var_of_event[e] = slot_of_var.at(plan.event_pdag_index[e]);
+    for (int32_t k = 0; k < n_samples; ++k)
         for (uint32_t e = 0; e < u->n_ev; ++e) {
-            const int32_t vi = var_of_event[e];
-            double pe = p[vi];
+            const double pe = p_samples[size_t(k) * size_t(pdag->n_vars) + size_t(u->var_of_event[e])];
             if (!(pe == pe)) return fail(MCS_ERR_INVALID_ARGUMENT, "sampled probability is NaN");
-            pe = pe > 1.0 ? 1.0 : (pe < 0.0 ? 0.0 : pe);
-            ev[size_t(k) * u->n_ev + e] = mcs::make_event_desc(plan.event_pdag_index[e], e, pe);
         }
-    }
-    std::vector<uint32_t> keys(static_cast<size_t>(n_samples));
-    for (int32_t k = 0; k < n_samples; ++k) {
-        const uint32_t id = sample_ids ? sample_ids[k] : uint32_t(k);
-        keys[size_t(k)] = u->cfg.common_random_numbers ? u->cfg.key0 : u->cfg.key0 + id;
-    }
 
     // Device.
     MCS_HIP(hipGetDevice(&u->device));
@@ -328,7 +384,7 @@ mcs_status mcs_ua_create(const mcs_pdag *pdag, const mcs_sample_shape *shape, in
     const uint64_t ones_n = 2ull * u->n_ev + uint64_t(n_samples) * u->slots;
     size_t free_b = 0, total_b = 0;
     MCS_HIP(hipMemGetInfo(&free_b, &total_b));
-    const uint64_t fixed = ev.size() * sizeof(mcs::EventDesc) + keys.size() * 4 +
+    const uint64_t fixed = uint64_t(n_samples) * u->n_ev * sizeof(mcs::EventDesc) + uint64_t(n_samples) * 4 +
                            u->lp.gates.size() * sizeof(mcs::GateDesc) +
                            (u->lp.args[0].size() + u->lp.args[1].size()) * 4 + ones_n * 8;
     const uint64_t reserve = uint64_t(256) << 20;
@@ -365,23 +421,14 @@ mcs_status mcs_ua_create(const mcs_pdag *pdag, const mcs_sample_shape *shape, in
     u->pipes.resize(n_pipes);
     for (uint32_t i = 0; i < n_pipes; ++i) {
         UaPipe &q = u->pipes[i];
-        q.k_begin = uint32_t(uint64_t(n_samples) * i / n_pipes);
-        q.k_end = uint32_t(uint64_t(n_samples) * (i + 1) / n_pipes);
         MCS_HIP(hipStreamCreateWithFlags(&q.stream, hipStreamNonBlocking));
         MCS_HIP(hipEventCreateWithFlags(&q.done, hipEventDisableTiming));
         MCS_HIP(hipMalloc(&q.d_words, word_bytes - sum_bytes));
         MCS_HIP(hipMalloc(&q.d_sum, sum_bytes));
-        MCS_HIP(hipMemsetAsync(q.d_sum, mcs::kChunkMixed, sum_bytes, u->stream));
     }
-    MCS_HIP(hipMalloc(&u->d_events, ev.size() * sizeof(mcs::EventDesc)));
-    MCS_HIP(hipMalloc(&u->d_keys, keys.size() * 4));
     MCS_HIP(hipMalloc(&u->d_gates, std::max<size_t>(u->lp.gates.size(), 1) * sizeof(mcs::GateDesc)));
     for (int a = 0; a < 2; ++a)
         MCS_HIP(hipMalloc(&u->d_args[a], std::max<size_t>(u->lp.args[a].size(), 1) * 4));
-    MCS_HIP(hipMalloc(&u->d_ones, ones_n * 8));
-    MCS_HIP(hipMemcpyAsync(u->d_events, ev.data(), ev.size() * sizeof(mcs::EventDesc),
-                           hipMemcpyHostToDevice, u->stream));
-    MCS_HIP(hipMemcpyAsync(u->d_keys, keys.data(), keys.size() * 4, hipMemcpyHostToDevice, u->stream));
     if (!u->lp.gates.empty())
         MCS_HIP(hipMemcpyAsync(u->d_gates, u->lp.gates.data(), u->lp.gates.size() * sizeof(mcs::GateDesc),
                                hipMemcpyHostToDevice, u->stream));
@@ -389,13 +436,26 @@ mcs_status mcs_ua_create(const mcs_pdag *pdag, const mcs_sample_shape *shape, in
         if (!u->lp.args[a].empty())
             MCS_HIP(hipMemcpyAsync(u->d_args[a], u->lp.args[a].data(), u->lp.args[a].size() * 4,
                                    hipMemcpyHostToDevice, u->stream));
-    MCS_HIP(hipMemsetAsync(u->d_ones, 0, ones_n * 8, u->stream));
-    MCS_HIP(hipStreamSynchronize(u->stream));
+    const mcs_status st = load_samples(u.get(), n_samples, p_samples, sample_ids);
+    if (st != MCS_OK) return st;
     u->device_bytes = need;
     // The counters of sample k are at d_ones + 2 n_ev + k * slots; the kernel
     // indexes them by row, and gate rows start at 2 n_ev.
     *out = u.release();
     return MCS_OK;
+}
+
+mcs_status mcs_ua_set_samples(mcs_ua *ua, int32_t n_samples, const double *p_samples,
+                              const uint32_t *sample_ids) {
+    if (!ua || !p_samples || n_samples <= 0)
+        return fail(MCS_ERR_INVALID_ARGUMENT, "null argument or no samples");
+    for (int32_t k = 0; k < n_samples; ++k)
+        for (uint32_t e = 0; e < ua->n_ev; ++e) {
+            const double pe = p_samples[size_t(k) * size_t(ua->n_vars) + size_t(ua->var_of_event[e])];
+            if (!(pe == pe)) return fail(MCS_ERR_INVALID_ARGUMENT, "sampled probability is NaN");
+        }
+    MCS_HIP(hipSetDevice(ua->device));
+    return load_samples(ua, n_samples, p_samples, sample_ids);
 }
 
 mcs_status mcs_ua_info(const mcs_ua *ua, uint32_t *samples_per_launch, uint32_t *pipelines,

@@ -130,6 +130,7 @@ def load_library():
         "mcs_batch_time_passes": [vp, u32, vp],
         "mcs_ua_create": [vp, vp, i32, vp, vp, vp, i32, vp, vp],
         "mcs_ua_destroy": [vp],
+        "mcs_ua_set_samples": [vp, i32, vp, vp],
         "mcs_ua_run": [vp, u32],
         "mcs_ua_collect": [vp, vp, C.c_int64, vp],
         "mcs_ua_time_run": [vp, u32, vp],

@@ -69,12 +69,42 @@ def independence_logits(pd: Pdag, p_samples) -> np.ndarray:
 def monte_carlo(pd: Pdag, p_samples, words: int, ids, passes: int = 1):
     """Top-event estimate of each parameter sample by plain GPU Monte Carlo
     (sample k uses Philox key (key0 + ids[k], key1)). -> (p_hat, seconds)."""
-    t = time.perf_counter()
-    with UncertaintyEngine(pd, words, p_samples, sample_ids=ids) as eng:
-        eng.run(passes)
-        ones, total = eng.collect()
-    dt = time.perf_counter() - t
-    return (ones[:, 0].astype(np.float64) + 0.5) / (total + 1.0), dt
+    with MonteCarloRunner(pd, words) as mc:
+        return mc(p_samples, ids, passes)
+
+
+class MonteCarloRunner:
+    """Plain GPU Monte Carlo of successive sets of parameter samples of one
+    tree through one uncertainty engine: its device arenas are allocated for
+    the first set and reused (``UncertaintyEngine.set_samples``), since a
+    large arena allocated again after a free costs seconds. Each call gives
+    the results a fresh engine would, bit for bit."""
+
+    def __init__(self, pd: Pdag, words: int):
+        self.pd, self.words, self.eng = pd, words, None
+
+    def __call__(self, p_samples, ids, passes: int = 1):
+        """-> (top-event estimate per sample, seconds)."""
+        t = time.perf_counter()
+        if self.eng is None:
+            self.eng = UncertaintyEngine(self.pd, self.words, p_samples, sample_ids=ids)
+        else:
+            self.eng.set_samples(p_samples, ids)
+        self.eng.run(passes)
+        ones, total = self.eng.collect()
+        dt = time.perf_counter() - t
+        return (ones[:, 0].astype(np.float64) + 0.5) / (total + 1.0), dt
+
+    def close(self):
+        if self.eng is not None:
+            self.eng.close()
+            self.eng = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
 
 
 @dataclass
@@ -122,17 +152,18 @@ def train_surrogate(pd: Pdag, model: LognormalModel, n_train: int, words: int, s
     base = np.zeros(n_train)
     z = np.empty((n_train, pd.n_vars), dtype=np.float32)
     t_mc = t_base = 0.0
-    for a, p, zc in model.draw_chunks(n_train, seed, chunk):
-        b = a + p.shape[0]
-        z[a:b] = zc
-        p_hat[a:b], dt = monte_carlo(pd, p, words, ids[a:b])
-        t_mc += dt
-        if baseline:
-            t = time.perf_counter()
-            base[a:b] = independence_logits(pd, p)
-            t_base += time.perf_counter() - t
-        if progress:
-            progress(b, n_train)
+    with MonteCarloRunner(pd, words) as mc:
+        for a, p, zc in model.draw_chunks(n_train, seed, chunk):
+            b = a + p.shape[0]
+            z[a:b] = zc
+            p_hat[a:b], dt = mc(p, ids[a:b])
+            t_mc += dt
+            if baseline:
+                t = time.perf_counter()
+                base[a:b] = independence_logits(pd, p)
+                t_base += time.perf_counter() - t
+            if progress:
+                progress(b, n_train)
     t = time.perf_counter()
     clip = float(np.log(2.0 * words * 64 + 1.0))
     ens = Ensemble(pd.n_vars, hidden=hidden, members=2, seed=seed,
@@ -202,12 +233,13 @@ def screened_uncertainty(pd: Pdag, model: LognormalModel, ens: Ensemble, n_outer
     if idx.size:
         p_r = np.concatenate(routed_p)
         p_hat = np.empty(idx.size)
-        for a in range(0, idx.size, mc_chunk):
-            b = min(idx.size, a + mc_chunk)
-            p_hat[a:b], dt = monte_carlo(pd, p_r[a:b], words, (id_base + idx[a:b]).astype(np.uint32))
-            t_mc += dt
-            if progress:
-                progress(b, idx.size)
+        with MonteCarloRunner(pd, words) as mc:
+            for a in range(0, idx.size, mc_chunk):
+                b = min(idx.size, a + mc_chunk)
+                p_hat[a:b], dt = mc(p_r[a:b], (id_base + idx[a:b]).astype(np.uint32))
+                t_mc += dt
+                if progress:
+                    progress(b, idx.size)
         surrogate_vals = est[idx].copy()
         est[idx] = p_hat
         a = audited[idx] & ~disagreed[idx]
@@ -229,12 +261,13 @@ def plain_uncertainty(pd: Pdag, model: LognormalModel, n_outer: int, words: int,
     ids = (id_base + np.arange(n_outer)).astype(np.uint32)
     est = np.empty(n_outer)
     seconds = 0.0
-    for a in range(0, n_outer, chunk):
-        b = min(n_outer, a + chunk)
-        est[a:b], dt = monte_carlo(pd, p[a:b], words, ids[a:b])
-        seconds += dt
-        if progress:
-            progress(b, n_outer)
+    with MonteCarloRunner(pd, words) as mc:
+        for a in range(0, n_outer, chunk):
+            b = min(n_outer, a + chunk)
+            est[a:b], dt = mc(p[a:b], ids[a:b])
+            seconds += dt
+            if progress:
+                progress(b, n_outer)
     return est, seconds
 
 

@@ -86,6 +86,20 @@ class UncertaintyEngine:
         except Exception:
             pass
 
+    def set_samples(self, p_samples, sample_ids=None) -> None:
+        """Replace the samples (mcs_ua_set_samples): the engine keeps its
+        device arenas and behaves as one created with these samples."""
+        p = np.ascontiguousarray(p_samples, dtype=np.float64)
+        if p.ndim != 2 or p.shape[1] != self.p.shape[1]:
+            raise ValueError("p_samples must be [n_samples, n_vars]")
+        ids = None if sample_ids is None else np.ascontiguousarray(sample_ids, dtype=np.uint32)
+        if ids is not None and ids.shape != (p.shape[0],):
+            raise ValueError("sample_ids must hold one id per sample")
+        self.p, self._ids = p, ids
+        self.n_samples = int(p.shape[0])
+        _check(load_library().mcs_ua_set_samples(self._h, self.n_samples, _p(p),
+                                                 _p(ids) if ids is not None else None))
+
     def run(self, n_passes: int = 1) -> None:
         _check(load_library().mcs_ua_run(self._h, n_passes))
 

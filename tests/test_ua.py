@@ -159,6 +159,36 @@ def test_ua_samples_per_launch_change_no_bit(gpu, tree):
 
 
 @pytest.mark.gpu
+def test_ua_set_samples_equals_a_fresh_engine(gpu):
+    # One engine loads three sets of samples in turn (7, then 11 on other
+    # ids, then 3), on two pipelines with several samples per launch, and
+    # runs each for two passes: every sample's tallies equal the oracle run
+    # with its own key, as a fresh engine's do, and the trial count restarts.
+    pd = small_random(35, n_be=40, n_gates=60, n_layers=6)
+    rng = np.random.default_rng(9)
+    tally = [pd.root, int(pd.gate_index[5]), int(pd.var_index[2])]
+    sets = []
+    for K, base in ((7, 10), (11, 900), (3, 40)):
+        P = np.clip(pd.var_p[None, :] * np.exp(rng.standard_normal((K, pd.n_vars))), 0, 1)
+        sets.append((P, (base + 5 * np.arange(K)).astype(np.uint32)))
+    P0, ids0 = sets[0]
+    with ua.UncertaintyEngine(pd, 1500, P0, sample_ids=ids0, tally=tally, samples_per_launch=2,
+                              pipelines=2) as eng:
+        for P, ids in sets:
+            if P is not P0:
+                eng.set_samples(P, ids)
+            eng.run(1)
+            eng.run(1)
+            ones, total = eng.collect()
+            assert total == 2 * 1500 * 64
+            assert ones.shape == (P.shape[0], len(tally))
+            for k in range(P.shape[0]):
+                want = _oracle_sample(pd, P[k], 1500, ((KEY[0] + int(ids[k])) & 0xFFFFFFFF, KEY[1]),
+                                      2, tally)
+                assert [int(v) for v in ones[k]] == [int(v) for v in want], (P.shape[0], k)
+
+
+@pytest.mark.gpu
 def test_ua_common_random_numbers(gpu):
     pd = small_random(32, n_be=30, n_gates=40, n_layers=5)
     P = np.stack([pd.var_p, pd.var_p, np.clip(pd.var_p * 2, 0, 1)])
