@@ -223,6 +223,9 @@ def test_null_handles():
     assert L.mcs_engine_collect(None, None, 0) == 1
     assert L.mcs_engine_destroy(None) == 0
     assert L.mcs_probability_analysis(None, None, None, None, 0, None, 0, None, None) == 1
+    assert L.mcs_ua_run(None, 1) == 1
+    assert L.mcs_ua_set_samples(None, 1, None, None) == 1
+    assert L.mcs_lognormal_samples(None, None, 3, 1, 0, 4, None, None) == 0  # no output: no-op
 
 
 def test_settings_defaults_are_the_reference_defaults():
